@@ -1,0 +1,527 @@
+// C ABI of libhpl_hip.so: context, resident system and the GEFA/GESL schedule.
+//
+// This file replaces the OpenCL execution backend
+// (LINPACK/src/host/execution_types/execution_pcie.hpp:53-728).  Where the
+// reference launches one single-work-item kernel per block and round-trips
+// every panel through the host, here one step of the blocked right-looking
+// factorisation is four stream-ordered launches on device-resident data:
+//   lu (diagonal block) -> prep (inverses of 32x32 diagonal sub-systems)
+//   -> trsm (whole top and left strips) -> gemm (whole trailing matrix).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/hpl_hip.h"
+#include "kernels.hpp"
+
+struct hpl_ctx {
+    int device = 0;
+    hipStream_t s0 = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    std::string err;
+
+    // resident system
+    uint64_t n = 0, lda = 0;
+    float* dA = nullptr;
+    size_t capA = 0;
+    float* dA0 = nullptr;  // untouched copy (keep_original)
+    size_t capA0 = 0;
+    bool keep_orig = false;
+    bool have_orig = false;
+    float* db = nullptr;
+    float* dx = nullptr;
+    double* db64 = nullptr;
+    double* dxt = nullptr;
+    size_t capVec = 0;
+
+    // GEFA workspaces
+    uint32_t B = 0;             // block size of the resident factor (0: none)
+    float* dLt = nullptr;       // transposed left strip, B x n
+    size_t capLt = 0;
+    float* dDT = nullptr;       // transposed diagonal block, B x B
+    size_t capDT = 0;
+    float* dInv = nullptr;      // per block: 3 * (B/32) inverses of 32 x 32
+    size_t capInv = 0;
+    float* dInner = nullptr;    // recursion scratch for blocks > 128
+    size_t capInner = 0;
+    int* dStatus = nullptr;
+    double* dWork = nullptr;    // residual partial sums
+    size_t capWork = 0;
+    double* dOut4 = nullptr;
+    bool factored = false;
+
+    // roofline accounting of the trailing update
+    bool prof = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+    size_t ev_used = 0;
+    double prof_flops = 0.0;
+};
+
+namespace {
+
+constexpr int kMaxLeafLU = 128;
+
+hpl_status fail(hpl_ctx* c, hpl_status st, const char* fmt, ...) {
+    if (c) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        c->err = buf;
+    }
+    return st;
+}
+
+#define HIPCHK(ctx, expr)                                                                 \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return fail((ctx), e_ == hipErrorOutOfMemory ? HPL_ERR_OUT_OF_MEMORY : HPL_ERR_HIP, \
+                        "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, \
+                        __LINE__);                                                        \
+    } while (0)
+
+template <typename T>
+hpl_status ensure(hpl_ctx* c, T*& p, size_t& cap, size_t bytes) {
+    if (bytes <= cap && p) return HPL_SUCCESS;
+    if (p) {
+        (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    if (bytes == 0) return HPL_SUCCESS;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), bytes);
+    if (e != hipSuccess) {
+        p = nullptr;
+        (void)hipGetLastError();
+        return fail(c, HPL_ERR_OUT_OF_MEMORY, "hipMalloc of %zu bytes failed: %s", bytes,
+                    hipGetErrorString(e));
+    }
+    cap = bytes;
+    return HPL_SUCCESS;
+}
+
+bool valid_block(uint64_t n, uint32_t B) {
+    if (B < 32 || B > 512 || (B & (B - 1))) return false;
+    return n > 0 && n % B == 0;
+}
+
+uint64_t device_lda(uint64_t n) {
+    uint64_t lda = (n + 63) / 64 * 64;
+    if (const char* e = getenv("HPL_LDA_PAD")) lda += (uint64_t)atoll(e);
+    return lda;
+}
+
+hpl_status alloc_system(hpl_ctx* c, uint64_t n) {
+    const uint64_t lda = device_lda(n);
+    hpl_status st = ensure(c, c->dA, c->capA, n * lda * sizeof(float));
+    if (st) return st;
+    if (c->keep_orig) {
+        st = ensure(c, c->dA0, c->capA0, n * lda * sizeof(float));
+        if (st) return st;
+    }
+    const size_t vb = n * sizeof(double);
+    if (vb > c->capVec) {
+        for (void* p : {(void*)c->db, (void*)c->dx, (void*)c->db64, (void*)c->dxt})
+            if (p) (void)hipFree(p);
+        c->db = c->dx = nullptr;
+        c->db64 = c->dxt = nullptr;
+        c->capVec = 0;
+        HIPCHK(c, hipMalloc(&c->db, n * sizeof(float)));
+        HIPCHK(c, hipMalloc(&c->dx, n * sizeof(float)));
+        HIPCHK(c, hipMalloc(&c->db64, vb));
+        HIPCHK(c, hipMalloc(&c->dxt, vb));
+        c->capVec = vb;
+    }
+    c->n = n;
+    c->lda = lda;
+    c->factored = false;
+    c->have_orig = false;
+    c->B = 0;
+    return HPL_SUCCESS;
+}
+
+hpl_status alloc_factor_ws(hpl_ctx* c, uint32_t B) {
+    const uint64_t n = c->n;
+    const uint64_t N = n / B;
+    hpl_status st = ensure(c, c->dLt, c->capLt, (size_t)B * n * sizeof(float));
+    if (st) return st;
+    st = ensure(c, c->dDT, c->capDT, (size_t)B * B * sizeof(float));
+    if (st) return st;
+    const size_t inv_stride = 3 * (B / 32) * 1024;
+    st = ensure(c, c->dInv, c->capInv, N * inv_stride * sizeof(float));
+    if (st) return st;
+    if (B > kMaxLeafLU) {
+        // recursion scratch per level h = B/2, B/4, .., 128: Lt and DT (h x h
+        // each) and the 3 h/32 inverses of 32 x 32
+        size_t need = 0;
+        for (uint32_t h = B / 2; h >= (uint32_t)kMaxLeafLU; h /= 2)
+            need += 2 * (size_t)h * h + 3 * (h / 32) * 1024;
+        st = ensure(c, c->dInner, c->capInner, need * sizeof(float));
+        if (st) return st;
+    }
+    if (!c->dStatus) HIPCHK(c, hipMalloc(&c->dStatus, 64));
+    return HPL_SUCCESS;
+}
+
+hpl_status launch_gemm(hpl_ctx* c, float* C, int64_t ldc, const float* Lt, int64_t ldl,
+                       const float* T, int64_t ldt, int M, int N, int K) {
+    if (M <= 0 || N <= 0) return HPL_SUCCESS;
+    hipEvent_t a = nullptr, b = nullptr;
+    if (c->prof) {
+        if (c->ev_used == c->ev.size()) {
+            hipEvent_t x, y;
+            HIPCHK(c, hipEventCreate(&x));
+            HIPCHK(c, hipEventCreate(&y));
+            c->ev.emplace_back(x, y);
+        }
+        a = c->ev[c->ev_used].first;
+        b = c->ev[c->ev_used].second;
+        c->ev_used++;
+        HIPCHK(c, hipEventRecord(a, c->s0));
+    }
+    HIPCHK(c, launch_gemm_update_f32(C, ldc, Lt, ldl, T, ldt, M, N, K, c->s0));
+    if (c->prof) {
+        HIPCHK(c, hipEventRecord(b, c->s0));
+        c->prof_flops += 2.0 * M * (double)N * K;
+    }
+    return HPL_SUCCESS;
+}
+
+// Factorise the B x B diagonal block at (off, off) in place.  Blocks above the
+// leaf size are split 2 x 2: LU(D11), solve D12/D21, D22 -= D21 D12, LU(D22).
+hpl_status factor_diag(hpl_ctx* c, int64_t off, uint32_t B, float* scratch) {
+    if (B <= (uint32_t)kMaxLeafLU) {
+        HIPCHK(c, launch_lu_diag(c->dA, c->lda, off, B, c->dStatus, c->s0));
+        return HPL_SUCCESS;
+    }
+    const uint32_t h = B / 2;
+    float* Lt = scratch;
+    float* DT = Lt + (size_t)h * h;
+    float* inv = DT + (size_t)h * h;
+    float* next = inv + 3 * (h / 32) * 1024;
+    hpl_status st = factor_diag(c, off, h, next);
+    if (st) return st;
+    HIPCHK(c, launch_prep(c->dA, c->lda, off, h, DT, inv, c->s0));
+    HIPCHK(c, launch_trsm(c->dA, c->lda, off, h, h, DT, inv, Lt, h, c->s0));
+    const int64_t lda = (int64_t)c->lda;
+    HIPCHK(c, launch_gemm_update_f32(c->dA + (off + h) * lda + off + h, lda, Lt, h,
+                                     c->dA + off * lda + off + h, lda, h, h, h, c->s0));
+    return factor_diag(c, off + h, h, next);
+}
+
+hpl_status gefa_schedule(hpl_ctx* c, uint32_t B) {
+    const int64_t n = (int64_t)c->n, lda = (int64_t)c->lda;
+    const int64_t N = n / B;
+    const int64_t inv_stride = 3 * (B / 32) * 1024;
+    for (int64_t k = 0; k < N; ++k) {
+        const int64_t off = k * B;
+        hpl_status st = factor_diag(c, off, B, c->dInner);
+        if (st) return st;
+        float* inv = c->dInv + k * inv_stride;
+        HIPCHK(c, launch_prep(c->dA, lda, off, B, c->dDT, inv, c->s0));
+        const int64_t m = n - off - B;
+        if (m <= 0) continue;
+        HIPCHK(c, launch_trsm(c->dA, lda, off, m, B, c->dDT, inv, c->dLt, n, c->s0));
+        st = launch_gemm(c, c->dA + (off + B) * lda + off + B, lda, c->dLt, n,
+                         c->dA + off * lda + off + B, lda, (int)m, (int)m, (int)B);
+        if (st) return st;
+    }
+    return HPL_SUCCESS;
+}
+
+hpl_status check_status(hpl_ctx* c) {
+    int h = 0;
+    HIPCHK(c, hipMemcpy(&h, c->dStatus, sizeof(int), hipMemcpyDeviceToHost));
+    if (h) return fail(c, HPL_ERR_ZERO_PIVOT, "zero or non-finite pivot met during GEFA");
+    return HPL_SUCCESS;
+}
+
+double elapsed_s(hpl_ctx* c) {
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, c->e0, c->e1);
+    return ms * 1e-3;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* hpl_version(void) { return "hpl_hip 1.0 (gfx950, FP32 MFMA)"; }
+
+const char* hpl_status_string(hpl_status s) {
+    switch (s) {
+        case HPL_SUCCESS: return "success";
+        case HPL_ERR_INVALID_ARG: return "invalid argument";
+        case HPL_ERR_HIP: return "HIP runtime error";
+        case HPL_ERR_OUT_OF_MEMORY: return "out of device memory";
+        case HPL_ERR_ZERO_PIVOT: return "zero or non-finite pivot";
+        case HPL_ERR_BUDGET: return "HBM budget exceeded";
+        case HPL_ERR_STATE: return "call out of order";
+        case HPL_ERR_NOT_CONVERGED: return "refinement did not converge";
+        case HPL_ERR_UNSUPPORTED: return "unsupported";
+        case HPL_ERR_NO_DEVICE: return "no HIP device";
+    }
+    return "unknown status";
+}
+
+hpl_status hpl_create(int device, hpl_ctx** out) {
+    if (!out) return HPL_ERR_INVALID_ARG;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return HPL_ERR_NO_DEVICE;
+    if (device >= count) return HPL_ERR_NO_DEVICE;
+    if (device < 0) device = 0 % count;
+    if (hipSetDevice(device) != hipSuccess) return HPL_ERR_NO_DEVICE;
+    hpl_ctx* c = new hpl_ctx();
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->s0, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->e0) != hipSuccess || hipEventCreate(&c->e1) != hipSuccess) {
+        hpl_destroy(c);
+        return HPL_ERR_HIP;
+    }
+    *out = c;
+    return HPL_SUCCESS;
+}
+
+void hpl_destroy(hpl_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->s0) (void)hipStreamSynchronize(c->s0);
+    for (void* p : {(void*)c->dA, (void*)c->dA0, (void*)c->db, (void*)c->dx, (void*)c->db64,
+                    (void*)c->dxt, (void*)c->dLt, (void*)c->dDT, (void*)c->dInv,
+                    (void*)c->dInner, (void*)c->dStatus, (void*)c->dWork, (void*)c->dOut4})
+        if (p) (void)hipFree(p);
+    for (auto& e : c->ev) {
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
+    }
+    if (c->e0) (void)hipEventDestroy(c->e0);
+    if (c->e1) (void)hipEventDestroy(c->e1);
+    if (c->s0) (void)hipStreamDestroy(c->s0);
+    delete c;
+}
+
+const char* hpl_last_error(const hpl_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+hpl_status hpl_device_name(hpl_ctx* c, char* buf, size_t len) {
+    if (!c || !buf || !len) return HPL_ERR_INVALID_ARG;
+    hipDeviceProp_t p;
+    HIPCHK(c, hipGetDeviceProperties(&p, c->device));
+    snprintf(buf, len, "%s (%s)", p.name, p.gcnArchName);
+    return HPL_SUCCESS;
+}
+
+hpl_status hpl_keep_original(hpl_ctx* c, int keep) {
+    if (!c) return HPL_ERR_INVALID_ARG;
+    c->keep_orig = keep != 0;
+    return HPL_SUCCESS;
+}
+
+hpl_status hpl_upload(hpl_ctx* c, const float* a, uint64_t n, uint64_t lda, const float* b,
+                      hpl_timing* t) {
+    if (!c || !a || n == 0 || lda < n) return fail(c, HPL_ERR_INVALID_ARG, "bad upload arguments");
+    HIPCHK(c, hipSetDevice(c->device));
+    hpl_status st = alloc_system(c, n);
+    if (st) return st;
+    HIPCHK(c, hipEventRecord(c->e0, c->s0));
+    HIPCHK(c, hipMemcpy2DAsync(c->dA, c->lda * sizeof(float), a, lda * sizeof(float),
+                               n * sizeof(float), n, hipMemcpyHostToDevice, c->s0));
+    if (b)
+        HIPCHK(c, hipMemcpyAsync(c->db, b, n * sizeof(float), hipMemcpyHostToDevice, c->s0));
+    else
+        HIPCHK(c, hipMemsetAsync(c->db, 0, n * sizeof(float), c->s0));
+    if (c->keep_orig) {
+        HIPCHK(c, hipMemcpyAsync(c->dA0, c->dA, n * c->lda * sizeof(float),
+                                 hipMemcpyDeviceToDevice, c->s0));
+        c->have_orig = true;
+    }
+    HIPCHK(c, hipEventRecord(c->e1, c->s0));
+    HIPCHK(c, hipStreamSynchronize(c->s0));
+    if (t) t->h2d_s = elapsed_s(c);
+    return HPL_SUCCESS;
+}
+
+hpl_status hpl_generate(hpl_ctx* c, uint64_t n, int mode, uint64_t seed, double d) {
+    if (!c || n == 0 || (mode != 0 && mode != 1))
+        return fail(c, HPL_ERR_INVALID_ARG, "bad generate arguments (n=%llu mode=%d)",
+                    (unsigned long long)n, mode);
+    HIPCHK(c, hipSetDevice(c->device));
+    hpl_status st = alloc_system(c, n);
+    if (st) return st;
+    HIPCHK(c, launch_generate(c->dA, (int64_t)c->lda, (int64_t)n, mode, seed, d, c->db, c->db64,
+                              c->dxt, c->s0));
+    if (c->keep_orig) {
+        HIPCHK(c, hipMemcpyAsync(c->dA0, c->dA, n * c->lda * sizeof(float),
+                                 hipMemcpyDeviceToDevice, c->s0));
+        c->have_orig = true;
+    }
+    HIPCHK(c, hipStreamSynchronize(c->s0));
+    return HPL_SUCCESS;
+}
+
+hpl_status hpl_restore(hpl_ctx* c) {
+    if (!c) return HPL_ERR_INVALID_ARG;
+    if (!c->have_orig) return fail(c, HPL_ERR_STATE, "no original copy kept");
+    HIPCHK(c, hipMemcpyAsync(c->dA, c->dA0, c->n * c->lda * sizeof(float),
+                             hipMemcpyDeviceToDevice, c->s0));
+    HIPCHK(c, hipStreamSynchronize(c->s0));
+    c->factored = false;
+    return HPL_SUCCESS;
+}
+
+hpl_status hpl_gefa_dev(hpl_ctx* c, uint32_t block, hpl_prec prec, hpl_timing* t) {
+    if (!c) return HPL_ERR_INVALID_ARG;
+    if (!c->dA || c->n == 0) return fail(c, HPL_ERR_STATE, "no resident matrix");
+    if (!valid_block(c->n, block))
+        return fail(c, HPL_ERR_INVALID_ARG,
+                    "block size %u invalid for n=%llu (power of two in [32,512] dividing n)",
+                    block, (unsigned long long)c->n);
+    if (prec != HPL_PREC_FP32)
+        return fail(c, HPL_ERR_UNSUPPORTED, "precision %d not available in this entry point",
+                    (int)prec);
+    HIPCHK(c, hipSetDevice(c->device));
+    hpl_status st = alloc_factor_ws(c, block);
+    if (st) return st;
+    HIPCHK(c, hipMemsetAsync(c->dStatus, 0, sizeof(int), c->s0));
+    c->factored = false;
+    HIPCHK(c, hipEventRecord(c->e0, c->s0));
+    st = gefa_schedule(c, block);
+    if (st) return st;
+    HIPCHK(c, hipEventRecord(c->e1, c->s0));
+    HIPCHK(c, hipStreamSynchronize(c->s0));
+    if (t) t->gefa_s = elapsed_s(c);
+    st = check_status(c);
+    if (st) return st;
+    c->B = block;
+    c->factored = true;
+    return HPL_SUCCESS;
+}
+
+hpl_status hpl_gesl_dev(hpl_ctx* c, hpl_timing* t) {
+    if (!c) return HPL_ERR_INVALID_ARG;
+    if (!c->factored) return fail(c, HPL_ERR_STATE, "hpl_gesl_dev before a successful GEFA");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipEventRecord(c->e0, c->s0));
+    HIPCHK(c, hipMemcpyAsync(c->dx, c->db, c->n * sizeof(float), hipMemcpyDeviceToDevice, c->s0));
+    HIPCHK(c, launch_gesl(c->dA, (int64_t)c->lda, (int64_t)c->n, (int)c->B, c->dInv,
+                          3 * (c->B / 32) * 1024, c->dx, c->s0));
+    HIPCHK(c, hipEventRecord(c->e1, c->s0));
+    HIPCHK(c, hipStreamSynchronize(c->s0));
+    if (t) t->gesl_s = elapsed_s(c);
+    return HPL_SUCCESS;
+}
+
+hpl_status hpl_download(hpl_ctx* c, float* a, uint64_t lda, float* x, hpl_timing* t) {
+    if (!c) return HPL_ERR_INVALID_ARG;
+    if (!c->dA) return fail(c, HPL_ERR_STATE, "no resident system");
+    if (a && lda < c->n) return fail(c, HPL_ERR_INVALID_ARG, "lda < n");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipEventRecord(c->e0, c->s0));
+    if (a)
+        HIPCHK(c, hipMemcpy2DAsync(a, lda * sizeof(float), c->dA, c->lda * sizeof(float),
+                                   c->n * sizeof(float), c->n, hipMemcpyDeviceToHost, c->s0));
+    if (x) HIPCHK(c, hipMemcpyAsync(x, c->dx, c->n * sizeof(float), hipMemcpyDeviceToHost, c->s0));
+    HIPCHK(c, hipEventRecord(c->e1, c->s0));
+    HIPCHK(c, hipStreamSynchronize(c->s0));
+    if (t) t->d2h_s = elapsed_s(c);
+    return HPL_SUCCESS;
+}
+
+hpl_status hpl_download_vectors(hpl_ctx* c, float* b, double* b64, double* xtrue) {
+    if (!c) return HPL_ERR_INVALID_ARG;
+    if (!c->dA) return fail(c, HPL_ERR_STATE, "no resident system");
+    HIPCHK(c, hipSetDevice(c->device));
+    const size_t n = c->n;
+    if (b) HIPCHK(c, hipMemcpyAsync(b, c->db, n * sizeof(float), hipMemcpyDeviceToHost, c->s0));
+    if (b64)
+        HIPCHK(c, hipMemcpyAsync(b64, c->db64, n * sizeof(double), hipMemcpyDeviceToHost, c->s0));
+    if (xtrue)
+        HIPCHK(c, hipMemcpyAsync(xtrue, c->dxt, n * sizeof(double), hipMemcpyDeviceToHost, c->s0));
+    HIPCHK(c, hipStreamSynchronize(c->s0));
+    return HPL_SUCCESS;
+}
+
+hpl_status hpl_gefa(hpl_ctx* c, float* a, uint64_t n, uint64_t lda, uint32_t block,
+                    hpl_timing* t) {
+    if (!c) return HPL_ERR_INVALID_ARG;
+    if (!a || lda < n || !valid_block(n, block))
+        return fail(c, HPL_ERR_INVALID_ARG, "bad gefa arguments (n=%llu lda=%llu block=%u)",
+                    (unsigned long long)n, (unsigned long long)lda, block);
+    hpl_status st = hpl_upload(c, a, n, lda, nullptr, t);
+    if (st) return st;
+    st = hpl_gefa_dev(c, block, HPL_PREC_FP32, t);
+    if (st) return st;
+    return hpl_download(c, a, lda, nullptr, t);
+}
+
+hpl_status hpl_gesl(hpl_ctx* c, const float* lu, float* b, uint64_t n, uint64_t lda,
+                    hpl_timing* t) {
+    if (!c) return HPL_ERR_INVALID_ARG;
+    if (!lu || !b || lda < n || n == 0) return fail(c, HPL_ERR_INVALID_ARG, "bad gesl arguments");
+    // block size for the solve: the largest power of two <= 256 dividing n
+    uint32_t B = 256;
+    while (B >= 32 && n % B) B /= 2;
+    if (B < 32) return fail(c, HPL_ERR_UNSUPPORTED, "n=%llu is not a multiple of 32",
+                            (unsigned long long)n);
+    hpl_status st = hpl_upload(c, lu, n, lda, b, t);
+    if (st) return st;
+    st = alloc_factor_ws(c, B);
+    if (st) return st;
+    HIPCHK(c, launch_prep_all(c->dA, (int64_t)c->lda, (int)B, (int)(n / B), c->dInv,
+                              3 * (B / 32) * 1024, c->s0));
+    c->B = B;
+    c->factored = true;
+    st = hpl_gesl_dev(c, t);
+    if (st) return st;
+    HIPCHK(c, hipMemcpy(b, c->dx, n * sizeof(float), hipMemcpyDeviceToHost));
+    return HPL_SUCCESS;
+}
+
+hpl_status hpl_residual(hpl_ctx* c, double out[4]) {
+    if (!c || !out) return HPL_ERR_INVALID_ARG;
+    if (!c->have_orig) return fail(c, HPL_ERR_STATE, "hpl_residual needs hpl_keep_original(1)");
+    HIPCHK(c, hipSetDevice(c->device));
+    const int64_t n = (int64_t)c->n;
+    hpl_status st = ensure(c, c->dWork, c->capWork,
+                           2 * (size_t)residual_chunks(n) * n * sizeof(double));
+    if (st) return st;
+    if (!c->dOut4) HIPCHK(c, hipMalloc(&c->dOut4, 4 * sizeof(double)));
+    HIPCHK(c, launch_residual_f32x(c->dA0, (int64_t)c->lda, n, c->dx, c->db, c->dWork, nullptr,
+                                   c->dOut4, c->s0));
+    HIPCHK(c, hipMemcpyAsync(out, c->dOut4, 4 * sizeof(double), hipMemcpyDeviceToHost, c->s0));
+    HIPCHK(c, hipStreamSynchronize(c->s0));
+    return HPL_SUCCESS;
+}
+
+hpl_status hpl_set_profiling(hpl_ctx* c, int enable) {
+    if (!c) return HPL_ERR_INVALID_ARG;
+    c->prof = enable != 0;
+    c->ev_used = 0;
+    c->prof_flops = 0.0;
+    return HPL_SUCCESS;
+}
+
+hpl_status hpl_kernel_stats(hpl_ctx* c, uint64_t* launches, double* seconds, double* flops) {
+    if (!c) return HPL_ERR_INVALID_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->s0));
+    double s = 0.0;
+    for (size_t i = 0; i < c->ev_used; ++i) {
+        float ms = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&ms, c->ev[i].first, c->ev[i].second));
+        s += ms * 1e-3;
+    }
+    if (launches) *launches = c->ev_used;
+    if (seconds) *seconds = s;
+    if (flops) *flops = c->prof_flops;
+    return HPL_SUCCESS;
+}
+
+}  // extern "C"

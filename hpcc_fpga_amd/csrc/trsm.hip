@@ -1,0 +1,233 @@
+// Panel triangular solves: "top_update" and "left_update"
+// (LINPACK/src/device/hpl_torus_PCIE.cl:452-582 and :588-709).
+//
+// After the diagonal block D (row-major, B x B) of step k is factorised, the
+// row-major strip right of it ("top" blocks, M's L-panel) and the strip below
+// it ("left" blocks, M's U-panel) are solved.  Both are a forward substitution
+// applied to many independent vectors v of length B:
+//     y_i = (v_i + sum_{p<i} C[i][p] y_p) * s_i,   i = 0..B-1
+//   top : vectors = columns of the strip, C[i][p] = D[i][p], s_i = D[i][i]
+//         (closed form of LINPACK/tests/test_kernel_communication.cpp:776-789)
+//   left: vectors = rows of the strip,    C[i][p] = D[p][i], s_i = 1
+//         (closed form of test_kernel_communication.cpp:624-632)
+// The left strip is also written transposed (k-major) for the trailing GEMM,
+// as the reference's left_update writes its transposed copy
+// (hpl_torus_PCIE.cl:697-708).
+//
+// Blocking: rows are taken 32 at a time.  For block q the contribution of the
+// already solved rows is a 32 x 32q by 32q x 32 product on MFMA
+// (v_mfma_f32_32x32x2_f32), and the 32 x 32 diagonal system is applied as its
+// precomputed inverse G_q = (S^-1 - C_qq)^-1, also on MFMA.  One wavefront
+// owns 32 vectors for all B rows, so no workgroup barrier is needed.
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace {
+
+// ----------------------------------------------------------------------------
+// prep: transposed copy of D (coefficients of the top solve become coalesced)
+// and the 32 x 32 inverses of the diagonal sub-systems, stored transposed:
+//   inv + 0*NQ*1024 : G^T for top   (S^-1 - C)^-1 with C = D lower, s = diag D
+//   inv + 1*NQ*1024 : G^T for left  (I - C)^-1 with C[i][p] = D[p][i]
+//   inv + 2*NQ*1024 : H^T for the back substitution of GESL, H = U_qq^-1 with
+//                     U[i][p] = D[p][i] (p > i), 1/u_ii = -D[i][i]
+// ----------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void prep_kernel(const float* __restrict__ A, int64_t lda,
+                                                  int64_t off, int B, float* __restrict__ DT,
+                                                  float* __restrict__ inv, int64_t inv_stride) {
+    __shared__ float S[32][33];
+    __shared__ float X[32][33];  // X[i][j] = column j of the inverse
+    const int NQ = B / 32;
+    const int t = threadIdx.x;
+    int blk = blockIdx.x;
+    const float* D = A + off * lda + off;
+    inv += (int64_t)blockIdx.y * inv_stride;
+    if (gridDim.y > 1) D += (int64_t)blockIdx.y * (B * lda + B);  // all-blocks mode
+    if (blk >= 3 * NQ) {
+        if (DT == nullptr) return;
+        blk -= 3 * NQ;
+        const int tr = blk / NQ, tc = blk % NQ;
+        for (int e = t; e < 1024; e += 64) {
+            const int i = e / 32, j = e % 32;
+            S[i][j] = D[(int64_t)(tr * 32 + i) * lda + tc * 32 + j];
+        }
+        __syncthreads();
+        for (int e = t; e < 1024; e += 64) {
+            const int j = e / 32, i = e % 32;
+            DT[(int64_t)(tc * 32 + j) * B + tr * 32 + i] = S[i][j];
+        }
+        return;
+    }
+    const int type = blk / NQ, q = blk % NQ;
+    for (int e = t; e < 1024; e += 64) {
+        const int i = e / 32, j = e % 32;
+        S[i][j] = D[(int64_t)(q * 32 + i) * lda + q * 32 + j];
+    }
+    __syncthreads();
+    if (t < 32) {
+        const int j = t;
+        if (type == 0) {
+            for (int i = 0; i < j; ++i) X[i][j] = 0.0f;
+            X[j][j] = S[j][j];
+            for (int i = j + 1; i < 32; ++i) {
+                float a = 0.0f;
+                for (int p = j; p < i; ++p) a = fmaf(S[i][p], X[p][j], a);
+                X[i][j] = a * S[i][i];
+            }
+        } else if (type == 1) {
+            for (int i = 0; i < j; ++i) X[i][j] = 0.0f;
+            X[j][j] = 1.0f;
+            for (int i = j + 1; i < 32; ++i) {
+                float a = 0.0f;
+                for (int p = j; p < i; ++p) a = fmaf(S[p][i], X[p][j], a);
+                X[i][j] = a;
+            }
+        } else {
+            for (int i = j + 1; i < 32; ++i) X[i][j] = 0.0f;
+            X[j][j] = -S[j][j];
+            for (int i = j - 1; i >= 0; --i) {
+                float a = 0.0f;
+                for (int p = i + 1; p <= j; ++p) a = fmaf(S[p][i], X[p][j], a);
+                X[i][j] = a * S[i][i];
+            }
+        }
+    }
+    __syncthreads();
+    float* out = inv + (int64_t)(type * NQ + q) * 1024;
+    for (int e = t; e < 1024; e += 64) {
+        const int j = e / 32, i = e % 32;  // out[j][i] = X[i][j]
+        out[e] = X[i][j];
+    }
+}
+
+// ----------------------------------------------------------------------------
+// The solve.  Launch: one wave per workgroup, grid.x = ceil(m/32) vectors
+// groups for the top strip followed by the same count for the left strip.
+// ----------------------------------------------------------------------------
+template <int B>
+__global__ __launch_bounds__(64) void trsm_kernel(float* __restrict__ A, int64_t lda, int64_t off,
+                                                  int64_t m, const float* __restrict__ DT,
+                                                  const float* __restrict__ inv,
+                                                  float* __restrict__ Lt, int64_t ldt,
+                                                  int ngroups_top) {
+    constexpr int NQ = B / 32;
+    __shared__ float Y[B][32];
+    __shared__ float SZ[32][33];  // staging tile of the left strip, then z_q
+    auto& Stg = SZ;
+    auto& Z = SZ;
+    const int lane = threadIdx.x;
+    const int lr = lane & 31, lh = lane >> 5;
+    const bool top = (int)blockIdx.x < ngroups_top;
+    const int64_t v0 = (int64_t)(top ? blockIdx.x : blockIdx.x - ngroups_top) * 32;
+    const float* D = A + off * lda + off;
+    // coefficient matrix read as X[p][i]: top -> D^T, left -> D
+    const float* Xc = top ? DT : D;
+    const int64_t ldx = top ? (int64_t)B : lda;
+    const float* G = inv + (top ? 0 : NQ * 1024);
+    float* stripT = A + off * lda + off + B;          // top strip: rows off.., cols off+B..
+    float* stripL = A + (off + B) * lda + off;        // left strip: rows off+B.., cols off..
+
+    for (int q = 0; q < NQ; ++q) {
+        f32x16 acc;
+        // ---- load v_q in accumulator layout (row i = acc_row, vector = lr)
+        if (top) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int i = acc_row(r, lane);
+                acc[r] = stripT[(int64_t)(q * 32 + i) * lda + v0 + lr];
+            }
+        } else {
+            // coalesced 32 x 32 tile (vector rows) through LDS
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const int vr = (lane >> 3) + 8 * s, c4 = (lane & 7) * 4;
+                const f32x4 v = *reinterpret_cast<const f32x4*>(
+                    stripL + (v0 + vr) * lda + q * 32 + c4);
+                Stg[vr][c4 + 0] = v[0];
+                Stg[vr][c4 + 1] = v[1];
+                Stg[vr][c4 + 2] = v[2];
+                Stg[vr][c4 + 3] = v[3];
+            }
+            wave_lds_sync();
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[r] = Stg[lr][acc_row(r, lane)];
+        }
+        // ---- z_q = v_q + C[q rows][0:32q] * Y[0:32q]
+        for (int kp = 0; kp < 16 * q; ++kp) {
+            const float a = Xc[(int64_t)(2 * kp + lh) * ldx + q * 32 + lr];
+            const float b = Y[2 * kp + lh][lr];
+            acc = mfma32(a, b, acc);
+        }
+        // ---- y_q = G_q z_q
+        wave_lds_sync();  // staging reads done before z overwrites the tile
+#pragma unroll
+        for (int r = 0; r < 16; ++r) Z[acc_row(r, lane)][lr] = acc[r];
+        wave_lds_sync();
+        f32x16 y = {0};
+        const float* Gq = G + q * 1024;
+#pragma unroll
+        for (int kp = 0; kp < 16; ++kp) {
+            const float a = Gq[(2 * kp + lh) * 32 + lr];
+            const float b = Z[2 * kp + lh][lr];
+            y = mfma32(a, b, y);
+        }
+#pragma unroll
+        for (int r = 0; r < 16; ++r) Y[q * 32 + acc_row(r, lane)][lr] = y[r];
+        // ---- write back
+        if (top) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r)
+                stripT[(int64_t)(q * 32 + acc_row(r, lane)) * lda + v0 + lr] = y[r];
+        } else {
+#pragma unroll
+            for (int r = 0; r < 16; ++r)
+                Lt[(int64_t)(q * 32 + acc_row(r, lane)) * ldt + v0 + lr] = y[r];
+            wave_lds_sync();
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const int vr = (lane >> 3) + 8 * s, c4 = (lane & 7) * 4;
+                f32x4 v;
+                v[0] = Y[q * 32 + c4 + 0][vr];
+                v[1] = Y[q * 32 + c4 + 1][vr];
+                v[2] = Y[q * 32 + c4 + 2][vr];
+                v[3] = Y[q * 32 + c4 + 3][vr];
+                *reinterpret_cast<f32x4*>(stripL + (v0 + vr) * lda + q * 32 + c4) = v;
+            }
+        }
+        wave_lds_sync();
+    }
+}
+
+}  // namespace
+
+hipError_t launch_prep(const float* A, int64_t lda, int64_t off, int B, float* DT, float* inv,
+                       hipStream_t s) {
+    const int NQ = B / 32;
+    dim3 grid(3 * NQ + (DT ? NQ * NQ : 0), 1);
+    prep_kernel<<<grid, 64, 0, s>>>(A, lda, off, B, DT, inv, 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_prep_all(const float* A, int64_t lda, int B, int nblocks, float* inv,
+                           int64_t inv_stride, hipStream_t s) {
+    const int NQ = B / 32;
+    dim3 grid(3 * NQ, nblocks);
+    prep_kernel<<<grid, 64, 0, s>>>(A, lda, 0, B, nullptr, inv, inv_stride);
+    return hipGetLastError();
+}
+
+hipError_t launch_trsm(float* A, int64_t lda, int64_t off, int64_t m, int B, const float* DT,
+                       const float* inv, float* Lt, int64_t ldt, hipStream_t s) {
+    if (m <= 0) return hipSuccess;
+    const int groups = (int)(m / 32);
+    dim3 grid(2 * groups);
+    switch (B) {
+        case 32: trsm_kernel<32><<<grid, 64, 0, s>>>(A, lda, off, m, DT, inv, Lt, ldt, groups); break;
+        case 64: trsm_kernel<64><<<grid, 64, 0, s>>>(A, lda, off, m, DT, inv, Lt, ldt, groups); break;
+        case 128: trsm_kernel<128><<<grid, 64, 0, s>>>(A, lda, off, m, DT, inv, Lt, ldt, groups); break;
+        case 256: trsm_kernel<256><<<grid, 64, 0, s>>>(A, lda, off, m, DT, inv, Lt, ldt, groups); break;
+        case 512: trsm_kernel<512><<<grid, 64, 0, s>>>(A, lda, off, m, DT, inv, Lt, ldt, groups); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}

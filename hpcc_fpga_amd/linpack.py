@@ -1,0 +1,238 @@
+"""Python binding of libhpl_hip.so (ctypes over the C ABI in include/hpl_hip.h).
+
+Mirrors the reference's LINPACK execution interface
+(LINPACK/src/host/execution_types/execution_pcie.hpp:53-728 and
+LINPACK/src/host/linpack_data.cpp:166-229): the matrix is the row-major float32
+array ``a`` whose transpose M = a.T is factorised without pivoting; after GEFA
+``a`` holds the dgefa encoding (negated multipliers below the diagonal of M,
+u above, -1/u_kk on the diagonal) and GESL solves M x = b.
+
+There is no CPU fallback: importing works without a GPU, but every compute
+call raises HplError when the HIP library or device is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libhpl_hip.so")
+
+HPL_SUCCESS = 0
+STATUS_NAMES = {
+    0: "HPL_SUCCESS", 1: "HPL_ERR_INVALID_ARG", 2: "HPL_ERR_HIP", 3: "HPL_ERR_OUT_OF_MEMORY",
+    4: "HPL_ERR_ZERO_PIVOT", 5: "HPL_ERR_BUDGET", 6: "HPL_ERR_STATE",
+    7: "HPL_ERR_NOT_CONVERGED", 8: "HPL_ERR_UNSUPPORTED", 9: "HPL_ERR_NO_DEVICE",
+}
+PRECISIONS = {"fp32": 0, "bf16": 1, "fp16": 2, "fp8": 3}
+
+
+class HplError(RuntimeError):
+    """A non-zero hpl_status (the C++ host maps it to an exception too)."""
+
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+        self.message = message
+
+
+class _Timing(C.Structure):
+    _fields_ = [("gefa_s", C.c_double), ("gesl_s", C.c_double), ("h2d_s", C.c_double),
+                ("d2h_s", C.c_double)]
+
+
+@dataclass
+class Timing:
+    gefa_s: float = 0.0
+    gesl_s: float = 0.0
+    h2d_s: float = 0.0
+    d2h_s: float = 0.0
+
+
+_lib = None
+
+
+def load_library():
+    """Load libhpl_hip.so; raise HplError if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise HplError(2, f"HIP library not built: {LIB_PATH} (run __graft_entry__.build())")
+    L = C.CDLL(LIB_PATH)
+    P = C.c_void_p
+    u64, u32, i32, dbl = C.c_uint64, C.c_uint32, C.c_int, C.c_double
+    T = C.POINTER(_Timing)
+    sigs = {
+        "hpl_create": ([i32, C.POINTER(P)], i32),
+        "hpl_destroy": ([P], None),
+        "hpl_last_error": ([P], C.c_char_p),
+        "hpl_status_string": ([i32], C.c_char_p),
+        "hpl_device_name": ([P, C.c_char_p, C.c_size_t], i32),
+        "hpl_version": ([], C.c_char_p),
+        "hpl_gefa": ([P, P, u64, u64, u32, T], i32),
+        "hpl_gesl": ([P, P, P, u64, u64, T], i32),
+        "hpl_upload": ([P, P, u64, u64, P, T], i32),
+        "hpl_generate": ([P, u64, i32, u64, dbl], i32),
+        "hpl_keep_original": ([P, i32], i32),
+        "hpl_gefa_dev": ([P, u32, i32, T], i32),
+        "hpl_gesl_dev": ([P, T], i32),
+        "hpl_restore": ([P], i32),
+        "hpl_download": ([P, P, u64, P, T], i32),
+        "hpl_residual": ([P, C.POINTER(C.c_double)], i32),
+        "hpl_download_vectors": ([P, P, P, P], i32),
+        "hpl_set_profiling": ([P, i32], i32),
+        "hpl_kernel_stats": ([P, C.POINTER(u64), C.POINTER(dbl), C.POINTER(dbl)], i32),
+    }
+    for name, (args, res) in sigs.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = L
+    return L
+
+
+def _f32(a: np.ndarray) -> np.ndarray:
+    if a.dtype != np.float32 or not a.flags["C_CONTIGUOUS"]:
+        raise TypeError("expected a C-contiguous float32 array")
+    return a
+
+
+class Context:
+    """One HIP device context holding one resident linear system."""
+
+    def __init__(self, device: int = 0):
+        self._L = load_library()
+        self._c = C.c_void_p()
+        st = self._L.hpl_create(device, C.byref(self._c))
+        if st != HPL_SUCCESS:
+            raise HplError(st, self._L.hpl_status_string(st).decode())
+        self.n = 0
+
+    # -- plumbing
+    def _check(self, st: int):
+        if st != HPL_SUCCESS:
+            raise HplError(st, self._L.hpl_last_error(self._c).decode())
+
+    def close(self):
+        if self._c:
+            self._L.hpl_destroy(self._c)
+            self._c = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def device_name(self) -> str:
+        buf = C.create_string_buffer(256)
+        self._check(self._L.hpl_device_name(self._c, buf, 256))
+        return buf.value.decode()
+
+    # -- host-buffer path (drop-in for execution::pcie::calculate)
+    def gefa(self, a: np.ndarray, block: int) -> Timing:
+        """Factorise ``a`` (n x n row-major float32) in place."""
+        a = _f32(a)
+        t = _Timing()
+        self._check(self._L.hpl_gefa(self._c, a.ctypes.data, a.shape[0], a.shape[1], block,
+                                     C.byref(t)))
+        return Timing(t.gefa_s, t.gesl_s, t.h2d_s, t.d2h_s)
+
+    def gesl(self, lu: np.ndarray, b: np.ndarray) -> Timing:
+        """Solve M x = b with a GEFA factor; b is overwritten with x."""
+        lu, b = _f32(lu), _f32(b)
+        t = _Timing()
+        self._check(self._L.hpl_gesl(self._c, lu.ctypes.data, b.ctypes.data, lu.shape[0],
+                                     lu.shape[1], C.byref(t)))
+        return Timing(t.gefa_s, t.gesl_s, t.h2d_s, t.d2h_s)
+
+    # -- device-resident path
+    def keep_original(self, keep: bool = True):
+        self._check(self._L.hpl_keep_original(self._c, int(keep)))
+
+    def upload(self, a: np.ndarray, b: np.ndarray | None = None) -> Timing:
+        a = _f32(a)
+        t = _Timing()
+        bp = _f32(b).ctypes.data if b is not None else None
+        self._check(self._L.hpl_upload(self._c, a.ctypes.data, a.shape[0], a.shape[1], bp,
+                                       C.byref(t)))
+        self.n = a.shape[0]
+        return Timing(t.gefa_s, t.gesl_s, t.h2d_s, t.d2h_s)
+
+    def generate(self, n: int, mode: int = 0, seed: int = 42, d: float = 1.0):
+        self._check(self._L.hpl_generate(self._c, n, mode, seed, d))
+        self.n = n
+
+    def gefa_dev(self, block: int, prec: str = "fp32") -> Timing:
+        t = _Timing()
+        self._check(self._L.hpl_gefa_dev(self._c, block, PRECISIONS[prec], C.byref(t)))
+        return Timing(t.gefa_s, t.gesl_s, t.h2d_s, t.d2h_s)
+
+    def gesl_dev(self) -> Timing:
+        t = _Timing()
+        self._check(self._L.hpl_gesl_dev(self._c, C.byref(t)))
+        return Timing(t.gefa_s, t.gesl_s, t.h2d_s, t.d2h_s)
+
+    def restore(self):
+        self._check(self._L.hpl_restore(self._c))
+
+    def download(self, matrix: bool = True, solution: bool = True):
+        n = self.n
+        a = np.empty((n, n), np.float32) if matrix else None
+        x = np.empty(n, np.float32) if solution else None
+        t = _Timing()
+        self._check(self._L.hpl_download(self._c, a.ctypes.data if matrix else None, n,
+                                         x.ctypes.data if solution else None, C.byref(t)))
+        return a, x
+
+    def download_vectors(self):
+        """(b float32, b float64, x_true float64) of the resident system."""
+        n = self.n
+        b = np.empty(n, np.float32)
+        b64 = np.empty(n, np.float64)
+        xt = np.empty(n, np.float64)
+        self._check(self._L.hpl_download_vectors(self._c, b.ctypes.data, b64.ctypes.data,
+                                                 xt.ctypes.data))
+        return b, b64, xt
+
+    def residual(self):
+        """(||b - Mx||inf, ||M||inf, ||x||inf, ||b||inf) in FP64 on the device."""
+        out = (C.c_double * 4)()
+        self._check(self._L.hpl_residual(self._c, out))
+        return tuple(out)
+
+    def set_profiling(self, enable: bool = True):
+        self._check(self._L.hpl_set_profiling(self._c, int(enable)))
+
+    def kernel_stats(self):
+        """(launches, seconds, flops) of the trailing-update GEMM since profiling on."""
+        n, s, f = C.c_uint64(), C.c_double(), C.c_double()
+        self._check(self._L.hpl_kernel_stats(self._c, C.byref(n), C.byref(s), C.byref(f)))
+        return int(n.value), float(s.value), float(f.value)
+
+
+def gefa(a: np.ndarray, block: int, device: int = 0) -> np.ndarray:
+    """Return the dgefa-encoded factor of ``a`` (a copy), computed on the GPU."""
+    out = np.ascontiguousarray(a, np.float32).copy()
+    with Context(device) as ctx:
+        ctx.gefa(out, block)
+    return out
+
+
+def gesl(lu: np.ndarray, b: np.ndarray, device: int = 0) -> np.ndarray:
+    """Return x with M x = b for a dgefa-encoded factor ``lu``, on the GPU."""
+    x = np.ascontiguousarray(b, np.float32).copy()
+    with Context(device) as ctx:
+        ctx.gesl(np.ascontiguousarray(lu, np.float32), x)
+    return x

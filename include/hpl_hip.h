@@ -1,0 +1,143 @@
+/*
+ * hpl_hip.h -- C ABI of libhpl_hip.so, the MI355X (gfx950) execution backend
+ * of the LINPACK benchmark: blocked LU factorisation without pivoting (GEFA)
+ * followed by forward and back substitution (GESL).
+ *
+ * Matrix convention (identical to the reference): the n x n matrix is stored
+ * row-major, a[row*lda + col], and is factorised as the column-major matrix
+ * M = A^T, M(i,j) = a[j*lda + i].  The system solved is M x = b.  After GEFA
+ * the buffer holds the LINPACK dgefa encoding without pivoting: the
+ * strictly-lower part of M holds the negated multipliers -l_ik, the
+ * strictly-upper part holds u_kj and the diagonal holds -1/u_kk
+ * (reference: LINPACK/src/host/linpack_data.cpp:166-229,
+ *  LINPACK/src/device/hpl_torus_PCIE.cl:62-71).
+ *
+ * Errors: every call returns hpl_status (HPL_SUCCESS == 0) and never throws;
+ * hpl_last_error() gives the message of the last failure on a context.  This
+ * replaces the reference's ASSERT_CL -> OpenClException convention
+ * (shared/include/setup/fpga_setup.hpp:125, shared/setup/fpga_setup.cpp:112-122);
+ * the C++ host maps a non-zero status to an exception.
+ *
+ * Threading: a context is not thread-safe; use one context per host thread.
+ * The caller owns every host buffer; the context owns every device buffer and
+ * frees them in hpl_destroy().
+ */
+#ifndef HPL_HIP_H_
+#define HPL_HIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum hpl_status {
+    HPL_SUCCESS = 0,
+    HPL_ERR_INVALID_ARG = 1,    /* bad size, block size, pointer or mode      */
+    HPL_ERR_HIP = 2,            /* a HIP runtime call failed                   */
+    HPL_ERR_OUT_OF_MEMORY = 3,  /* device allocation failed                    */
+    HPL_ERR_ZERO_PIVOT = 4,     /* zero or non-finite pivot (no pivoting)      */
+    HPL_ERR_BUDGET = 5,         /* request exceeds the HBM admission budget    */
+    HPL_ERR_STATE = 6,          /* call out of order (e.g. solve before factor)*/
+    HPL_ERR_NOT_CONVERGED = 7,  /* iterative refinement missed its tolerance   */
+    HPL_ERR_UNSUPPORTED = 8,    /* valid request this build does not implement */
+    HPL_ERR_NO_DEVICE = 9       /* no HIP device, or device index out of range */
+} hpl_status;
+
+/* Input precision of the trailing-update GEMM.  Accumulation is always FP32;
+ * the panel LU and the triangular solves always run in FP32. */
+typedef enum hpl_prec {
+    HPL_PREC_FP32 = 0,
+    HPL_PREC_BF16 = 1,
+    HPL_PREC_FP16 = 2,
+    HPL_PREC_FP8 = 3 /* E4M3 with one FP32 scale per B x B operand block */
+} hpl_prec;
+
+/* Wall times of the last call, measured with HIP events on the library's
+ * stream (seconds).  Fields a call does not touch are left unchanged. */
+typedef struct hpl_timing {
+    double gefa_s;
+    double gesl_s;
+    double h2d_s;
+    double d2h_s;
+} hpl_timing;
+
+typedef struct hpl_ctx hpl_ctx;
+
+/* ---- context ------------------------------------------------------------ */
+
+/* Select device `device` (>= 0) or, for device < 0, device (rank % count) as
+ * the reference's round-robin selection does (shared/setup/fpga_setup.cpp:291-293),
+ * with rank taken from 0.  Replaces fpga_setup::selectFPGADevice +
+ * fpga_setup::fpgaSetup (shared/setup/fpga_setup.cpp:132-181,200-312). */
+hpl_status hpl_create(int device, hpl_ctx** ctx);
+void hpl_destroy(hpl_ctx* ctx);
+const char* hpl_last_error(const hpl_ctx* ctx);
+const char* hpl_status_string(hpl_status status);
+/* Device name as reported by the runtime (ExecutionSettings::getDeviceName). */
+hpl_status hpl_device_name(hpl_ctx* ctx, char* buf, size_t len);
+/* Library build string (version and target). */
+const char* hpl_version(void);
+
+/* ---- host-buffer path: the drop-in for execution::pcie::calculate -------
+ * (LINPACK/src/host/execution_types/execution_pcie.hpp:53-728).
+ * hpl_gefa uploads a, factorises it on the device and writes the factor back
+ * into a (in place, dgefa encoding), as calculate() leaves data.A
+ * (execution_pcie.hpp:695-703).  block must be a power of two in [32, 256]
+ * dividing n.  hpl_gesl solves M x = b with a factor produced by hpl_gefa
+ * (host buffer lu) and overwrites b with x, as gesl_ref_nopvt does
+ * (LINPACK/src/host/linpack_data.cpp:199-229). */
+hpl_status hpl_gefa(hpl_ctx* ctx, float* a, uint64_t n, uint64_t lda, uint32_t block,
+                    hpl_timing* timing);
+hpl_status hpl_gesl(hpl_ctx* ctx, const float* lu, float* b, uint64_t n, uint64_t lda,
+                    hpl_timing* timing);
+
+/* ---- device-resident path -------------------------------------------------
+ * The context holds one resident system (A, b, x).  Repetitions then time
+ * only device work, as the reference times GEFA after the upload
+ * (execution_pcie.hpp:123-127,162-163). */
+
+/* Copy a host system to the device (b may be NULL: then b = 0). */
+hpl_status hpl_upload(hpl_ctx* ctx, const float* a, uint64_t n, uint64_t lda, const float* b,
+                      hpl_timing* timing);
+/* Generate a system on the device with the counter-based generator:
+ *  mode 0: M(i,j) = U[-0.5,0.5) off the diagonal, M(i,i) += n, b = M*1 (FP64
+ *          sums rounded to FP32);
+ *  mode 1: M(i,i) = d * sum_{j!=i} |M(i,j)|, x_true = U[-1,1), b = M x_true
+ *          in FP64 (kept in FP64 for refinement, rounded copy in FP32).
+ * Element (r,c) of the row-major storage is a pure function of (seed, r*n+c),
+ * so the result is bit-identical to the CPU generator of the same name. */
+hpl_status hpl_generate(hpl_ctx* ctx, uint64_t n, int mode, uint64_t seed, double d);
+/* Keep (1) or drop (0) an untouched FP32 copy of A for residuals and
+ * refinement.  Must be set before hpl_upload/hpl_generate. */
+hpl_status hpl_keep_original(hpl_ctx* ctx, int keep);
+/* Factorise the resident matrix in place. */
+hpl_status hpl_gefa_dev(hpl_ctx* ctx, uint32_t block, hpl_prec prec, hpl_timing* timing);
+/* Solve with the resident factor: x = M^-1 b (b is kept). */
+hpl_status hpl_gesl_dev(hpl_ctx* ctx, hpl_timing* timing);
+/* Restore the resident matrix from the kept original (device copy). */
+hpl_status hpl_restore(hpl_ctx* ctx);
+/* Copy back the resident matrix (a may be NULL) and/or solution x (may be
+ * NULL).  lda is the host leading dimension. */
+hpl_status hpl_download(hpl_ctx* ctx, float* a, uint64_t lda, float* x, hpl_timing* timing);
+/* Copy back the resident right-hand side (b, FP32), its FP64 version (b64)
+ * and the generator's exact solution (xtrue, FP64); any pointer may be NULL. */
+hpl_status hpl_download_vectors(hpl_ctx* ctx, float* b, double* b64, double* xtrue);
+/* FP64 norms on the device against the kept original:
+ *  out[0] = ||b - M x||inf, out[1] = ||M||inf, out[2] = ||x||inf, out[3] = ||b||inf. */
+hpl_status hpl_residual(hpl_ctx* ctx, double out[4]);
+
+/* ---- kernel timing for roofline accounting -------------------------------
+ * When enabled, every trailing-update GEMM launch is bracketed by HIP events
+ * on the stream it runs on.  hpl_kernel_stats returns the number of launches
+ * since the last reset, their summed duration (s) and their algorithmic
+ * flops (2*m_rows*m_cols*B per launch). */
+hpl_status hpl_set_profiling(hpl_ctx* ctx, int enable);
+hpl_status hpl_kernel_stats(hpl_ctx* ctx, uint64_t* launches, double* seconds, double* flops);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HPL_HIP_H_ */
