@@ -1,0 +1,252 @@
+"""CPU checks that pin the oracle (no GPU).
+
+Each oracle function is checked against something other than itself:
+  generate_ref      - known answer derived from the reference README's emulator
+                      run (LINPACK/README.md:168-172: ||b||inf = resid /
+                      (residn * n * eps) = 35.88236618041992 at n=32, seed 0),
+                      an independent numpy MT19937 restatement (bit-exact),
+                      and the invariants of test_host_reference_implementations.cpp:30-53;
+  generate_sized    - independent numpy splitmix64 restatement (bit-exact),
+                      dominance margin, b = M 1 in FP64;
+  gefa_nopvt        - decoded factor reproduces M (numpy FP64), equals scipy's
+                      LU (no row exchange happens for these column-dominant
+                      matrices), brute force on 2x2/3x3, committed golden vector;
+  gesl_nopvt        - the reference's own dtrsm_ref (oracle/_ref) and scipy
+                      solve_triangular on the decoded factor;
+  *_f64             - numpy.linalg.solve, the reference's FGMRES (gmres_ref);
+  top/left/inner    - triangular-solve identities and a plain matmul;
+  gefa_blocked      - bit-identical to gefa_nopvt;
+  residn_ref/norms  - direct numpy evaluation of the reference formulas.
+"""
+import os
+
+import numpy as np
+import pytest
+import scipy.linalg as sla
+
+from oracle import oracle as O
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+# --------------------------------------------------------------------------- generators
+def test_ref_generator_readme_known_answer():
+    # LINPACK/README.md:168-172 (n=32, B=16 emulator run, rank 0 -> seed 0)
+    resid, residn, eps = 5.960464477539062e-07, 0.004354506590071576, 1.1920928955078125e-07
+    normb_readme = resid / (residn * 32 * eps)
+    _, b, _, normb = O.generate_ref(32, seed=0)
+    assert normb == pytest.approx(normb_readme, rel=1e-12)
+    assert float(np.max(b)) == normb
+
+
+def _numpy_ref_generator(n, seed):
+    """Independent restatement: numpy's MT19937 (init_genrand seeding, as
+    std::mt19937(seed)) + libstdc++ generate_canonical<double,53> with two
+    32-bit draws: u = (g1 + g2 * 2^32) / 2^64."""
+    rs = np.random.RandomState(seed)
+    g = rs.randint(0, 2 ** 32, size=2 * n * n, dtype=np.uint64).astype(np.float64)
+    u = (g[0::2] + g[1::2] * 4294967296.0) / 18446744073709551616.0
+    u = np.where(u >= 1.0, np.nextafter(1.0, 0.0), u)
+    A = u.astype(np.float32).reshape(n, n)
+    for r in range(n):
+        A[r, r] = 0.0
+        s = np.float32(0.0)
+        for v in A[r]:
+            s = np.float32(s + v)
+        A[r, r] = s
+    b = np.zeros(n, np.float32)
+    for j in range(n):
+        s = np.float32(0.0)
+        for v in A[:, j]:
+            s = np.float32(s + v)
+        b[j] = s
+    return A, b
+
+
+@pytest.mark.parametrize("n,seed", [(16, 0), (33, 1), (64, 5)])
+def test_ref_generator_matches_independent_mt19937(n, seed):
+    A, b, _, _ = O.generate_ref(n, seed=seed)
+    A2, b2 = _numpy_ref_generator(n, seed)
+    assert np.array_equal(A.view(np.uint32), A2.view(np.uint32))
+    assert np.array_equal(b.view(np.uint32), b2.view(np.uint32))
+
+
+def test_ref_generator_invariants():
+    # test_host_reference_implementations.cpp:30-53
+    A, b, norma, normb = O.generate_ref(128, seed=0, diag_dominant=False)
+    assert np.all((A > 0) & (A < 1))
+    A, b, norma, normb = O.generate_ref(128, seed=0)
+    off = A - np.diag(np.diag(A))
+    assert np.all((off[~np.eye(128, dtype=bool)] > 0) & (off[~np.eye(128, dtype=bool)] < 1))
+    assert np.allclose(np.diag(A), off.sum(axis=1, dtype=np.float64), rtol=1e-5)
+    assert norma == float(np.max(A))
+    x = np.linalg.solve(A.T.astype(np.float64), b.astype(np.float64))
+    assert np.max(np.abs(x - 1.0)) < 1e-5
+
+
+def _splitmix_uniform(seed, stream, k):
+    M = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+    def mix(z):
+        z = np.asarray(z, np.uint64)
+        with np.errstate(over="ignore"):
+            z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+            z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return (z ^ (z >> np.uint64(31))) & M
+
+    key = mix(np.uint64(seed) ^ (np.uint64(stream) << np.uint64(56)))
+    with np.errstate(over="ignore"):
+        z = mix(key + (np.asarray(k, np.uint64) + np.uint64(1)) * np.uint64(0x9E3779B97F4A7C15))
+    return (z >> np.uint64(40)).astype(np.float32) * np.float32(1.0 / 16777216.0)
+
+
+def test_sized_generator_matches_independent_splitmix():
+    n = 96
+    A, b, b64, xt = O.generate_sized(n, 42, 0)
+    U = _splitmix_uniform(42, 0, np.arange(n * n)).reshape(n, n) - np.float32(0.5)
+    U[np.arange(n), np.arange(n)] += np.float32(n)
+    assert np.array_equal(A.view(np.uint32), U.view(np.uint32))
+    assert O.hash_uniform(42, 0, 5) == float(_splitmix_uniform(42, 0, [5])[0])
+    # b = M 1 in FP64 (sum order differs from numpy: tolerance)
+    assert np.allclose(b64, A.T.astype(np.float64).sum(axis=1), rtol=1e-14, atol=1e-12)
+    assert np.array_equal(xt, np.ones(n))
+    # dominance margin: |A_ii| >= n - 0.5 > sum_{j!=i} |A_ij| <= 0.5 (n-1)
+    off = np.abs(A.T.astype(np.float64)).sum(axis=1) - np.abs(np.diag(A))
+    assert np.all(np.abs(np.diag(A)) > off)
+
+
+def test_sized_generator_mode1():
+    n, d = 80, 1.1
+    A, b, b64, xt = O.generate_sized(n, 7, 1, d)
+    M = A.T.astype(np.float64)
+    offs = np.abs(M).sum(axis=1) - np.abs(np.diag(M))
+    assert np.allclose(np.diag(M), d * offs, rtol=1e-6)
+    assert np.all((xt >= -1) & (xt < 1))
+    assert np.allclose(b64, M @ xt, rtol=1e-13, atol=1e-12)
+    assert np.array_equal(b, b64.astype(np.float32))
+    xs = _splitmix_uniform(7, 1, np.arange(n)).astype(np.float32)
+    assert np.array_equal(xt, (np.float32(2) * xs - np.float32(1)).astype(np.float64))
+
+
+# --------------------------------------------------------------------------- factor / solve
+def test_gefa_brute_force_2x2():
+    # M = [[a, c], [b, d]] stored row-major as A = M^T = [[a, b], [c, d]]
+    a, b_, c, d = 4.0, 1.0, 2.0, 5.0
+    A = np.array([[a, b_], [c, d]], np.float32)
+    F = O.gefa_nopvt(A)
+    l = b_ / a
+    u22 = d - l * c
+    assert F[0, 0] == np.float32(-1.0 / a)
+    assert F[0, 1] == pytest.approx(-l)          # -l stored below M's diagonal
+    assert F[1, 0] == c                          # u_12
+    assert F[1, 1] == pytest.approx(-1.0 / u22)
+
+
+@pytest.mark.parametrize("n", [3, 17, 64])
+def test_gefa_decoded_reproduces_matrix(n):
+    A, _, _, _ = O.generate_ref(n, seed=n)
+    F = O.gefa_nopvt_f64(A.astype(np.float64))
+    L, U = O.decode_factor(F)
+    M = A.T.astype(np.float64)
+    assert np.allclose(L @ U, M, rtol=1e-13, atol=1e-12 * np.abs(M).max())
+    P, Ls, Us = sla.lu(M)
+    assert np.array_equal(P, np.eye(n))  # column dominance: no row exchange
+    assert np.allclose(Ls, L, rtol=1e-12, atol=1e-14)
+    assert np.allclose(Us, U, rtol=1e-12, atol=1e-12)
+    F32 = O.gefa_nopvt(A)
+    assert np.max(np.abs(F32 - F)) <= 1e-5 * np.max(np.abs(F))
+
+
+def test_gesl_against_reference_dtrsm_and_scipy():
+    n = 48
+    A, b, _, normb = O.generate_ref(n, seed=2)
+    F = O.gefa_nopvt(A)
+    x = O.gesl_nopvt(F, b)
+    L, U = O.decode_factor(F)
+    y = sla.solve_triangular(L, b.astype(np.float64), lower=True, unit_diagonal=True)
+    xs = sla.solve_triangular(U, y, lower=False)
+    assert np.max(np.abs(x - xs)) < 1e-5
+    R = O.ref_lib()
+    if R is None:
+        pytest.skip("oracle/_ref not built (reference tree absent)")
+    # LAPACK-form factor, column-major (= our row-major transpose layout)
+    LU = np.ascontiguousarray((np.tril(L, -1) + U).T)  # col-major storage of L\U
+    r = b.astype(np.float64).copy()
+    R.dtrsm_ref(b"L", b"L", b"N", b"U", n, 1, 1.0, LU, n, r, n)
+    R.dtrsm_ref(b"L", b"U", b"N", b"N", n, 1, 1.0, LU, n, r, n)
+    assert np.max(np.abs(r - xs)) < 1e-10
+    _, residn = O.residn_ref(x, normb)
+    assert residn < 1.0
+
+
+def test_f64_solve_against_numpy_and_reference_gmres():
+    n = 64
+    A, b, b64, xt = O.generate_sized(n, 7, 1, 1.1)
+    M = A.T.astype(np.float64)
+    F = O.gefa_nopvt_f64(A.astype(np.float64))
+    x = O.gesl_nopvt_f64(F, b64)
+    assert np.allclose(x, np.linalg.solve(M, b64), rtol=1e-12, atol=1e-13)
+    assert np.allclose(x, xt, rtol=1e-10, atol=1e-11)
+    R = O.ref_lib()
+    if R is None:
+        pytest.skip("oracle/_ref not built (reference tree absent)")
+    L, U = O.decode_factor(O.gefa_nopvt(A))  # FP32 factor as preconditioner
+    LU = np.ascontiguousarray((np.tril(L, -1) + U).T)
+    Acm = np.ascontiguousarray(A.astype(np.float64))  # col-major M
+    xg = np.zeros(n)
+    R.gmres_ref(n, Acm, n, xg, b64.copy(), LU, n, 50, 1, 1e-14)
+    assert np.max(np.abs(xg - x)) < 1e-11
+
+
+def test_block_closed_forms_against_triangular_identities():
+    B = 32
+    D, _, _, _ = O.generate_ref(B, seed=11)
+    lu = O.gefa_nopvt(D)
+    L, U = O.decode_factor(lu.astype(np.float64))
+    rng = np.random.default_rng(0)
+    X = rng.random((B, B)).astype(np.float32)
+    top = O.top_update(X, lu)       # U^T top = -X  (rows of M's L-panel)
+    assert np.allclose(U.T @ top.astype(np.float64), -X, rtol=1e-4, atol=1e-5)
+    left = O.left_update(X, lu)     # L left^T = X^T  (M's U-panel)
+    assert np.allclose(L @ left.T.astype(np.float64), X.T, rtol=1e-4, atol=1e-5)
+    Cm = rng.random((B, B)).astype(np.float32)
+    inner = O.inner_update(Cm, left, top)
+    ref = Cm.astype(np.float64) + left.astype(np.float64) @ top.astype(np.float64)
+    bound = B * np.finfo(np.float32).eps * (np.abs(Cm) + np.abs(left) @ np.abs(top))
+    assert np.all(np.abs(inner - ref) <= bound + 1e-7)
+
+
+@pytest.mark.parametrize("n,B", [(64, 16), (96, 32), (128, 64)])
+def test_blocked_equals_unblocked(n, B):
+    A, _, _, _ = O.generate_ref(n, seed=1)
+    assert np.array_equal(O.gefa_blocked(A, B).view(np.uint32), O.gefa_nopvt(A).view(np.uint32))
+
+
+def test_metrics():
+    x = np.array([1.0, 1.5, 0.75], np.float32)
+    resid, residn = O.residn_ref(x, 2.0)
+    assert resid == 0.5
+    assert residn == pytest.approx(0.5 / (3 * 2.0 * np.finfo(np.float32).eps))
+    A, b, _, _ = O.generate_ref(20, seed=4)
+    xs = np.linspace(-1, 1, 20)
+    rn, an, xn, bn = O.norms(A, xs, b.astype(np.float64))
+    M = A.T.astype(np.float64)
+    assert rn == pytest.approx(np.max(np.abs(b - M @ xs)), rel=1e-12)
+    assert an == pytest.approx(np.max(np.abs(M).sum(axis=1)), rel=1e-12)
+    assert xn == 1.0 and bn == pytest.approx(float(np.max(np.abs(b))))
+
+
+# --------------------------------------------------------------------------- golden vectors
+def test_golden_vectors():
+    """Committed fixtures (tests/golden/make_golden.py): the oracle must keep
+    reproducing them bit for bit."""
+    path = os.path.join(GOLDEN, "linpack_golden.npz")
+    g = np.load(path, allow_pickle=False)
+    A, b, _, normb = O.generate_ref(32, seed=0)
+    assert np.array_equal(A, g["ref32_A"]) and np.array_equal(b, g["ref32_b"])
+    F = O.gefa_nopvt(A)
+    assert np.array_equal(F, g["ref32_factor"])
+    assert np.array_equal(O.gesl_nopvt(F, b), g["ref32_x"])
+    As, bs, _, _ = O.generate_sized(64, 42, 0)
+    assert np.array_equal(As, g["sized64_A"]) and np.array_equal(bs, g["sized64_b"])
+    assert float(g["readme_normb"]) == normb
