@@ -22,8 +22,11 @@
 
 struct hpl_ctx {
     int device = 0;
-    hipStream_t s0 = nullptr;
+    hipStream_t s0 = nullptr;   // trailing updates (and everything else)
+    hipStream_t s1 = nullptr;   // look-ahead panel factorisation
     hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipEvent_t evA = nullptr, evP = nullptr;  // next-panel region updated / panel ready
+    bool lookahead = true;
     std::string err;
 
     // resident system
@@ -42,9 +45,9 @@ struct hpl_ctx {
 
     // GEFA workspaces
     uint32_t B = 0;             // block size of the resident factor (0: none)
-    float* dLt = nullptr;       // transposed left strip, B x n
+    float* dLt = nullptr;       // transposed left strip, 2 x (B x n), by step parity
     size_t capLt = 0;
-    float* dDT = nullptr;       // transposed diagonal block, B x B
+    float* dDT = nullptr;       // transposed diagonal block, 2 x (B x B), by step parity
     size_t capDT = 0;
     float* dInv = nullptr;      // per block: 3 * (B/32) inverses of 32 x 32
     size_t capInv = 0;
@@ -151,9 +154,9 @@ hpl_status alloc_system(hpl_ctx* c, uint64_t n) {
 hpl_status alloc_factor_ws(hpl_ctx* c, uint32_t B) {
     const uint64_t n = c->n;
     const uint64_t N = n / B;
-    hpl_status st = ensure(c, c->dLt, c->capLt, (size_t)B * n * sizeof(float));
+    hpl_status st = ensure(c, c->dLt, c->capLt, 2 * (size_t)B * n * sizeof(float));
     if (st) return st;
-    st = ensure(c, c->dDT, c->capDT, (size_t)B * B * sizeof(float));
+    st = ensure(c, c->dDT, c->capDT, 2 * (size_t)B * B * sizeof(float));
     if (st) return st;
     const size_t inv_stride = 3 * (B / 32) * 1024;
     st = ensure(c, c->dInv, c->capInv, N * inv_stride * sizeof(float));
@@ -171,8 +174,8 @@ hpl_status alloc_factor_ws(hpl_ctx* c, uint32_t B) {
     return HPL_SUCCESS;
 }
 
-hpl_status launch_gemm(hpl_ctx* c, float* C, int64_t ldc, const float* Lt, int64_t ldl,
-                       const float* T, int64_t ldt, int M, int N, int K) {
+hpl_status launch_gemm(hpl_ctx* c, hipStream_t st, float* C, int64_t ldc, const float* Lt,
+                       int64_t ldl, const float* T, int64_t ldt, int M, int N, int K) {
     if (M <= 0 || N <= 0) return HPL_SUCCESS;
     hipEvent_t a = nullptr, b = nullptr;
     if (c->prof) {
@@ -185,11 +188,11 @@ hpl_status launch_gemm(hpl_ctx* c, float* C, int64_t ldc, const float* Lt, int64
         a = c->ev[c->ev_used].first;
         b = c->ev[c->ev_used].second;
         c->ev_used++;
-        HIPCHK(c, hipEventRecord(a, c->s0));
+        HIPCHK(c, hipEventRecord(a, st));
     }
-    HIPCHK(c, launch_gemm_update_f32(C, ldc, Lt, ldl, T, ldt, M, N, K, c->s0));
+    HIPCHK(c, launch_gemm_update_f32(C, ldc, Lt, ldl, T, ldt, M, N, K, st));
     if (c->prof) {
-        HIPCHK(c, hipEventRecord(b, c->s0));
+        HIPCHK(c, hipEventRecord(b, st));
         c->prof_flops += 2.0 * M * (double)N * K;
     }
     return HPL_SUCCESS;
@@ -197,9 +200,9 @@ hpl_status launch_gemm(hpl_ctx* c, float* C, int64_t ldc, const float* Lt, int64
 
 // Factorise the B x B diagonal block at (off, off) in place.  Blocks above the
 // leaf size are split 2 x 2: LU(D11), solve D12/D21, D22 -= D21 D12, LU(D22).
-hpl_status factor_diag(hpl_ctx* c, int64_t off, uint32_t B, float* scratch) {
+hpl_status factor_diag(hpl_ctx* c, hipStream_t st, int64_t off, uint32_t B, float* scratch) {
     if (B <= (uint32_t)kMaxLeafLU) {
-        HIPCHK(c, launch_lu_diag(c->dA, c->lda, off, B, c->dStatus, c->s0));
+        HIPCHK(c, launch_lu_diag(c->dA, c->lda, off, B, c->dStatus, st));
         return HPL_SUCCESS;
     }
     const uint32_t h = B / 2;
@@ -207,32 +210,71 @@ hpl_status factor_diag(hpl_ctx* c, int64_t off, uint32_t B, float* scratch) {
     float* DT = Lt + (size_t)h * h;
     float* inv = DT + (size_t)h * h;
     float* next = inv + 3 * (h / 32) * 1024;
-    hpl_status st = factor_diag(c, off, h, next);
-    if (st) return st;
-    HIPCHK(c, launch_prep(c->dA, c->lda, off, h, DT, inv, c->s0));
-    HIPCHK(c, launch_trsm(c->dA, c->lda, off, h, h, DT, inv, Lt, h, c->s0));
+    hpl_status r = factor_diag(c, st, off, h, next);
+    if (r) return r;
+    HIPCHK(c, launch_prep(c->dA, c->lda, off, h, DT, inv, st));
+    HIPCHK(c, launch_trsm(c->dA, c->lda, off, h, h, DT, inv, Lt, h, st));
     const int64_t lda = (int64_t)c->lda;
     HIPCHK(c, launch_gemm_update_f32(c->dA + (off + h) * lda + off + h, lda, Lt, h,
-                                     c->dA + off * lda + off + h, lda, h, h, h, c->s0));
-    return factor_diag(c, off + h, h, next);
+                                     c->dA + off * lda + off + h, lda, h, h, h, st));
+    return factor_diag(c, st, off + h, h, next);
 }
 
+// Panel of step k: factor the diagonal block, prepare its inverses, solve the
+// top and left strips (the left one also into Lt[k % 2]).
+hpl_status panel(hpl_ctx* c, hipStream_t st, int64_t k, uint32_t B) {
+    const int64_t n = (int64_t)c->n, lda = (int64_t)c->lda;
+    const int64_t off = k * B;
+    hpl_status r = factor_diag(c, st, off, B, c->dInner);
+    if (r) return r;
+    float* inv = c->dInv + k * (int64_t)(3 * (B / 32) * 1024);
+    float* DT = c->dDT + (k & 1) * (int64_t)B * B;
+    HIPCHK(c, launch_prep(c->dA, lda, off, B, DT, inv, st));
+    const int64_t m = n - off - B;
+    if (m > 0)
+        HIPCHK(c, launch_trsm(c->dA, lda, off, m, B, DT, inv, c->dLt + (k & 1) * (int64_t)B * n,
+                              n, st));
+    return HPL_SUCCESS;
+}
+
+// Blocked right-looking schedule with depth-1 look-ahead (the pattern of the
+// reference's XRT backend, execution_xrt_pcie.hpp:328-415): the trailing
+// update of step k first updates the next panel's row and column blocks, the
+// panel of step k+1 then runs on stream s1 while stream s0 updates the rest.
 hpl_status gefa_schedule(hpl_ctx* c, uint32_t B) {
     const int64_t n = (int64_t)c->n, lda = (int64_t)c->lda;
     const int64_t N = n / B;
-    const int64_t inv_stride = 3 * (B / 32) * 1024;
-    for (int64_t k = 0; k < N; ++k) {
-        const int64_t off = k * B;
-        hpl_status st = factor_diag(c, off, B, c->dInner);
-        if (st) return st;
-        float* inv = c->dInv + k * inv_stride;
-        HIPCHK(c, launch_prep(c->dA, lda, off, B, c->dDT, inv, c->s0));
-        const int64_t m = n - off - B;
-        if (m <= 0) continue;
-        HIPCHK(c, launch_trsm(c->dA, lda, off, m, B, c->dDT, inv, c->dLt, n, c->s0));
-        st = launch_gemm(c, c->dA + (off + B) * lda + off + B, lda, c->dLt, n,
-                         c->dA + off * lda + off + B, lda, (int)m, (int)m, (int)B);
-        if (st) return st;
+    hpl_status r = panel(c, c->s0, 0, B);
+    if (r) return r;
+    for (int64_t k = 0; k + 1 < N; ++k) {
+        const int64_t k1 = (k + 1) * B;  // first trailing row/column
+        const int64_t m = n - k1;
+        const float* Lt = c->dLt + (k & 1) * (int64_t)B * n;
+        const float* T = c->dA + k * B * lda + k1;  // top strip of step k
+        float* C = c->dA + k1 * lda + k1;
+        if (!c->lookahead || m <= (int64_t)B) {
+            r = launch_gemm(c, c->s0, C, lda, Lt, n, T, lda, (int)m, (int)m, (int)B);
+            if (r) return r;
+            r = panel(c, c->s0, k + 1, B);
+            if (r) return r;
+            continue;
+        }
+        // next panel's region: row block k+1 and column block k+1
+        r = launch_gemm(c, c->s0, C, lda, Lt, n, T, lda, (int)B, (int)m, (int)B);
+        if (r) return r;
+        r = launch_gemm(c, c->s0, C + B * lda, lda, Lt + B, n, T, lda, (int)(m - B), (int)B,
+                        (int)B);
+        if (r) return r;
+        HIPCHK(c, hipEventRecord(c->evA, c->s0));
+        HIPCHK(c, hipStreamWaitEvent(c->s1, c->evA, 0));
+        r = panel(c, c->s1, k + 1, B);
+        if (r) return r;
+        HIPCHK(c, hipEventRecord(c->evP, c->s1));
+        // the rest of the trailing matrix
+        r = launch_gemm(c, c->s0, C + B * lda + B, lda, Lt + B, n, T + B, lda, (int)(m - B),
+                        (int)(m - B), (int)B);
+        if (r) return r;
+        HIPCHK(c, hipStreamWaitEvent(c->s0, c->evP, 0));
     }
     return HPL_SUCCESS;
 }
@@ -282,8 +324,12 @@ hpl_status hpl_create(int device, hpl_ctx** out) {
     if (hipSetDevice(device) != hipSuccess) return HPL_ERR_NO_DEVICE;
     hpl_ctx* c = new hpl_ctx();
     c->device = device;
+    if (const char* e = getenv("HPL_LOOKAHEAD")) c->lookahead = atoi(e) != 0;
     if (hipStreamCreateWithFlags(&c->s0, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&c->e0) != hipSuccess || hipEventCreate(&c->e1) != hipSuccess) {
+        hipStreamCreateWithFlags(&c->s1, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->e0) != hipSuccess || hipEventCreate(&c->e1) != hipSuccess ||
+        hipEventCreateWithFlags(&c->evA, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->evP, hipEventDisableTiming) != hipSuccess) {
         hpl_destroy(c);
         return HPL_ERR_HIP;
     }
@@ -295,6 +341,7 @@ void hpl_destroy(hpl_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->s0) (void)hipStreamSynchronize(c->s0);
+    if (c->s1) (void)hipStreamSynchronize(c->s1);
     for (void* p : {(void*)c->dA, (void*)c->dA0, (void*)c->db, (void*)c->dx, (void*)c->db64,
                     (void*)c->dxt, (void*)c->dLt, (void*)c->dDT, (void*)c->dInv,
                     (void*)c->dInner, (void*)c->dStatus, (void*)c->dWork, (void*)c->dOut4})
@@ -305,6 +352,9 @@ void hpl_destroy(hpl_ctx* c) {
     }
     if (c->e0) (void)hipEventDestroy(c->e0);
     if (c->e1) (void)hipEventDestroy(c->e1);
+    if (c->evA) (void)hipEventDestroy(c->evA);
+    if (c->evP) (void)hipEventDestroy(c->evP);
+    if (c->s1) (void)hipStreamDestroy(c->s1);
     if (c->s0) (void)hipStreamDestroy(c->s0);
     delete c;
 }

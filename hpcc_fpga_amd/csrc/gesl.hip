@@ -5,7 +5,8 @@
 // With D_q the q-th diagonal block and A the row-major factor:
 //   forward  (L y = b, L unit lower, -l stored):  x_e += sum_{p in q} A[p][e] y_p
 //   backward (U x = y):                           x_e -= sum_{p in q} A[p][e] x_p
-// Both read rows of the factor, so every update is a coalesced GEMV.
+// Both read rows of the factor, so every update is a coalesced GEMV (float4
+// loads, the p range split over thread groups and reduced in LDS).
 //
 // One launch per block step.  Workgroup 0 first applies the just-solved block
 // to the NEXT diagonal block's entries and then solves that diagonal system;
@@ -19,10 +20,42 @@
 namespace {
 
 constexpr int NT = 256;
-constexpr int CHUNK = 64;  // entries per non-diagonal workgroup
+constexpr int CHUNK = 256;  // entries per non-diagonal workgroup (64 float4 columns)
 
-// Solve the B x B diagonal system of block Qb in x (in place), forward
-// (lower, unit) or backward (upper).  256 threads.
+// red[0][e] (e < 4*ncols) = sum_{p < B} A[(pbase + p) * lda + ebase + e] * v[p] for the
+// entries ebase .. ebase + 4*ncols - 1 below elim (ncols <= 64).
+__device__ __forceinline__ void block_gemv(const float* __restrict__ A, int64_t lda, int64_t pbase,
+                                           int64_t ebase, int ncols, int B, const float* v,
+                                           float (*red)[4 * 64], int64_t elim) {
+    const int t = threadIdx.x;
+    const int groups = min(NT / ncols, NT / 16);  // p-groups (rows of red)
+    const int col = t % ncols, g = t / ncols;
+    f32x4 a = {0, 0, 0, 0};
+    const int64_t e = ebase + 4 * col;
+    if (g < groups && e < elim) {
+        const float* src = A + pbase * lda + e;
+#pragma unroll 8
+        for (int p = g; p < B; p += groups) {
+            const f32x4 m = *reinterpret_cast<const f32x4*>(src + (int64_t)p * lda);
+            const float vp = v[p];
+            a[0] = fmaf(m[0], vp, a[0]);
+            a[1] = fmaf(m[1], vp, a[1]);
+            a[2] = fmaf(m[2], vp, a[2]);
+            a[3] = fmaf(m[3], vp, a[3]);
+        }
+    }
+    if (g < groups) *reinterpret_cast<f32x4*>(&red[g][4 * col]) = a;
+    __syncthreads();
+    if (t < 4 * ncols) {
+        float s = 0.0f;
+        for (int gg = 0; gg < groups; ++gg) s += red[gg][t];
+        red[0][t] = s;
+    }
+    __syncthreads();
+}
+
+// Solve the B x B diagonal system starting at Qoff in x (in place), forward
+// (unit lower) or backward (upper).  256 threads.
 template <bool FWD>
 __device__ void diag_solve(const float* __restrict__ A, int64_t lda, int64_t Qoff, int B,
                            const float* __restrict__ invT, float* __restrict__ x, float* ybuf,
@@ -38,8 +71,9 @@ __device__ void diag_solve(const float* __restrict__ A, int64_t lda, int64_t Qof
         const int p_lo = FWD ? 0 : 32 * (s + 1);
         const int p_hi = FWD ? 32 * s : B;
         float a = 0.0f;
-        for (int p = p_lo + g; p < p_hi; p += 8)
-            a = fmaf(A[(Qoff + p) * lda + Qoff + 32 * s + i], ybuf[p], a);
+        const float* col = A + Qoff * lda + Qoff + 32 * s + i;
+#pragma unroll 4
+        for (int p = p_lo + g; p < p_hi; p += 8) a = fmaf(col[(int64_t)p * lda], ybuf[p], a);
         part[g][i] = a;
         __syncthreads();
         if (t < 32) {
@@ -54,6 +88,7 @@ __device__ void diag_solve(const float* __restrict__ A, int64_t lda, int64_t Qof
             // y_s = X z with X^T stored: X[t][j] = invT[j][t]
             float y = 0.0f;
             const float* X = invT + s * 1024;
+#pragma unroll 8
             for (int j = 0; j < 32; ++j) y = fmaf(X[j * 32 + t], part[0][j], y);
             ybuf[32 * s + t] = y;
         }
@@ -70,53 +105,44 @@ __global__ __launch_bounds__(NT) void gesl_fwd_kernel(const float* __restrict__ 
     __shared__ float ybuf[512];
     __shared__ float part[8][32];
     __shared__ float ys[512];
+    __shared__ float red[NT / 16][4 * 64];
     const int t = threadIdx.x;
     const int NQ = B / 32;
+    const int64_t qoff = (int64_t)q * B;
     if (q >= 0) {
-        for (int e = t; e < B; e += NT) ys[e] = x[(int64_t)q * B + e];
+        for (int e = t; e < B; e += NT) ys[e] = x[qoff + e];
         __syncthreads();
     }
-    const int64_t qoff = (int64_t)q * B;
     if (blockIdx.x == 0) {
         const int64_t Q = (int64_t)(q + 1) * B;
         if (q >= 0) {
             // entries of block q+1: x_e += sum_p A[qoff+p][e] y_p
-            for (int e = t; e < B; e += NT) {
-                float a = 0.0f;
-                for (int p = 0; p < B; ++p) a = fmaf(A[(qoff + p) * lda + Q + e], ys[p], a);
-                x[Q + e] += a;
+            for (int e0 = 0; e0 < B; e0 += 256) {
+                const int ncols = min(64, (B - e0) / 4);
+                block_gemv(A, lda, qoff, Q + e0, ncols, B, ys, red, n);
+                if (t < 4 * ncols) x[Q + e0 + t] += red[0][t];
+                __syncthreads();
             }
-            __syncthreads();
         }
         diag_solve<true>(A, lda, Q, B, inv + (int64_t)(q + 1) * inv_stride + NQ * 1024, x, ybuf,
                          part);
         return;
     }
-    // entries [(q+2)B, n) in chunks of CHUNK; 4 partial sums per entry
+    // entries [(q+2)B, n) in chunks of CHUNK
     const int64_t e0 = (int64_t)(q + 2) * B + (int64_t)(blockIdx.x - 1) * CHUNK;
-    const int e = t % CHUNK, g = t / CHUNK;
-    float a = 0.0f;
-    if (e0 + e < n)
-        for (int p = g; p < B; p += NT / CHUNK) a = fmaf(A[(qoff + p) * lda + e0 + e], ys[p], a);
-    __shared__ float red[NT / CHUNK][CHUNK];
-    red[g][e] = a;
-    __syncthreads();
-    if (t < CHUNK && e0 + t < n) {
-        float s = 0.0f;
-#pragma unroll
-        for (int gg = 0; gg < NT / CHUNK; ++gg) s += red[gg][t];
-        x[e0 + t] += s;
-    }
+    block_gemv(A, lda, qoff, e0, 64, B, ys, red, n);
+    if (t < CHUNK && e0 + t < n) x[e0 + t] += red[0][t];
 }
 
 // Backward step q (q = N .. 1): apply solved block q (if q < N), solve q-1.
 __global__ __launch_bounds__(NT) void gesl_bwd_kernel(const float* __restrict__ A, int64_t lda,
-                                                      int64_t n, int B, int q, int N,
+                                                      int B, int q, int N,
                                                       const float* __restrict__ inv,
                                                       int64_t inv_stride, float* __restrict__ x) {
     __shared__ float ybuf[512];
     __shared__ float part[8][32];
     __shared__ float xs[512];
+    __shared__ float red[NT / 16][4 * 64];
     const int t = threadIdx.x;
     const int NQ = B / 32;
     const int64_t qoff = (int64_t)q * B;
@@ -127,12 +153,12 @@ __global__ __launch_bounds__(NT) void gesl_bwd_kernel(const float* __restrict__ 
     if (blockIdx.x == 0) {
         const int64_t Q = (int64_t)(q - 1) * B;
         if (q < N) {
-            for (int e = t; e < B; e += NT) {
-                float a = 0.0f;
-                for (int p = 0; p < B; ++p) a = fmaf(A[(qoff + p) * lda + Q + e], xs[p], a);
-                x[Q + e] -= a;
+            for (int e0 = 0; e0 < B; e0 += 256) {
+                const int ncols = min(64, (B - e0) / 4);
+                block_gemv(A, lda, qoff, Q + e0, ncols, B, xs, red, Q + B);
+                if (t < 4 * ncols) x[Q + e0 + t] -= red[0][t];
+                __syncthreads();
             }
-            __syncthreads();
         }
         diag_solve<false>(A, lda, Q, B, inv + (int64_t)(q - 1) * inv_stride + 2 * NQ * 1024, x,
                           ybuf, part);
@@ -141,20 +167,8 @@ __global__ __launch_bounds__(NT) void gesl_bwd_kernel(const float* __restrict__ 
     // entries [0, (q-1)B)
     const int64_t e0 = (int64_t)(blockIdx.x - 1) * CHUNK;
     const int64_t lim = (int64_t)(q - 1) * B;
-    const int e = t % CHUNK, g = t / CHUNK;
-    float a = 0.0f;
-    if (e0 + e < lim)
-        for (int p = g; p < B; p += NT / CHUNK) a = fmaf(A[(qoff + p) * lda + e0 + e], xs[p], a);
-    __shared__ float red[NT / CHUNK][CHUNK];
-    red[g][e] = a;
-    __syncthreads();
-    if (t < CHUNK && e0 + t < lim) {
-        float s = 0.0f;
-#pragma unroll
-        for (int gg = 0; gg < NT / CHUNK; ++gg) s += red[gg][t];
-        x[e0 + t] -= s;
-    }
-    (void)n;
+    block_gemv(A, lda, qoff, e0, 64, B, xs, red, lim);
+    if (t < CHUNK && e0 + t < lim) x[e0 + t] -= red[0][t];
 }
 
 }  // namespace
@@ -171,7 +185,7 @@ hipError_t launch_gesl(const float* A, int64_t lda, int64_t n, int B, const floa
     for (int q = N; q >= 1; --q) {
         const int64_t rest = (int64_t)(q - 1) * B;
         const int grid = 1 + (q < N ? (int)((rest + CHUNK - 1) / CHUNK) : 0);
-        gesl_bwd_kernel<<<grid, NT, 0, s>>>(A, lda, n, B, q, N, inv, inv_stride, x);
+        gesl_bwd_kernel<<<grid, NT, 0, s>>>(A, lda, B, q, N, inv, inv_stride, x);
     }
     return hipGetLastError();
 }

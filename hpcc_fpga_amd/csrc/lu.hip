@@ -5,95 +5,131 @@
 // block (LINPACK/src/host/linpack_data.cpp:166-196):
 //   for gk: D[gk][gk] = -1/D[gk][gk]; D[gk][c] *= D[gk][gk] (c > gk);
 //           D[r][c] += D[gk][c] * D[r][gk]  (r > gk, c > gk).
-// Every element sees the same multiply and add in the same order as the
-// reference loop, so the block result is bit-identical to it.
+// Every element receives the same multiplies and adds in the same order as in
+// the reference loop (no FMA contraction), so the result is bit-identical.
 //
+// The block lives in LDS and is processed in panels of 32 pivot rows:
+//   1. wave 0 eliminates the 32 panel rows (all columns right of the panel
+//      start) with the panel held in registers; pivots and multipliers are
+//      broadcast with v_readlane, so the 32 steps need no barrier;
+//   2. every row below the panel is solved against the panel's 32 x 32
+//      diagonal block (the "left" closed form), one row per thread;
+//   3. the trailing part receives the panel's 32 rank-1 updates in k order,
+//      4 x 4 register tiles per thread.
+// Three workgroup barriers per panel instead of one or two per pivot.
 // Blocks up to 128 are handled here; larger diagonal blocks are factorised by
 // the host schedule as a 2 x 2 recursion on top of this kernel.
-//
-// One workgroup of NT threads holds the block in registers: thread t owns
-// column c = t % B and rows rg + RG*i (rg = t / B, RG = NT / B).  Each step
-// publishes the pivot column and pivot row of the NEXT step through LDS
-// (double-buffered), so a step costs one workgroup barrier.
 #include "common.hpp"
 #include "kernels.hpp"
 
 namespace {
 
-template <int B, int NT>
+constexpr int NT = 256;
+
+__device__ __forceinline__ float bcast(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+template <int B>
 __global__ __launch_bounds__(NT) void lu_diag_kernel(float* __restrict__ A, int64_t lda,
                                                       int64_t off, int* __restrict__ status) {
-    // separate multiply and add, rounded as the reference loop rounds them
 #pragma clang fp contract(off)
-    constexpr int RG = NT / B;  // row groups
-    constexpr int R = B / RG;   // rows per thread
-    static_assert(RG * B == NT && R * RG == B, "bad LU shape");
-    // colbuf[p][rg*R + i] = D[rg + RG*i][pivot column]; rowbuf[p][c] = D[pivot row][c]
-    __shared__ __attribute__((aligned(16))) float colbuf[2][B];
-    __shared__ float rowbuf[2][B];
-
+    constexpr int LD = B + 1;  // padded LDS row
+    __shared__ float Ds[B * LD];
     const int t = threadIdx.x;
-    const int c = t % B;
-    const int rg = t / B;
+    const int lane = t & 63, wave = t >> 6;
     float* base = A + off * lda + off;
 
-    float d[R];
-#pragma unroll
-    for (int i = 0; i < R; ++i) d[i] = base[(int64_t)(rg + RG * i) * lda + c];
-
-    // publish step 0: column 0 and row 0
-    if (c == 0) {
-#pragma unroll
-        for (int i = 0; i < R; ++i) colbuf[0][rg * R + i] = d[i];
+    for (int e = t; e < B * B; e += NT) {
+        const int r = e / B, c = e % B;
+        Ds[r * LD + c] = base[(int64_t)r * lda + c];
     }
-    if (rg == 0) rowbuf[0][c] = d[0];
+    __syncthreads();
 
     bool bad = false;
+    for (int P0 = 0; P0 < B; P0 += 32) {
+        // ---- 1. panel rows [P0, P0+32), columns [P0, B): wave 0, registers
+        if (wave == 0) {
+            const int c0 = P0 + lane, c1 = P0 + 64 + lane;
+            const bool h0 = c0 < B, h1 = c1 < B;
+            float d0[32], d1[32];
 #pragma unroll
-    for (int i = 0; i < R; ++i) {
-        for (int rr = 0; rr < RG; ++rr) {
-            const int gk = i * RG + rr;
-            const int p = gk & 1;
-            __syncthreads();
-            const float piv = colbuf[p][rr * R + i];
-            const float inv = -1.0f / piv;
-            bad |= !(__builtin_isfinite(inv)) || piv == 0.0f;
-            const float rv = rowbuf[p][c] * inv;  // scaled pivot-row element (c > gk)
-            if (rg == rr) d[i] = (c > gk) ? rv : ((c == gk) ? inv : d[i]);
-            if (c > gk) {
-                // rows r > gk: register i when rg > rr, every register above i.
-                // Chunks of 4 with float4 LDS reads; the scheduling barrier
-                // keeps the compiler from hoisting every read (register cap).
+            for (int r = 0; r < 32; ++r) {
+                d0[r] = h0 ? Ds[(P0 + r) * LD + c0] : 0.0f;
+                d1[r] = h1 ? Ds[(P0 + r) * LD + c1] : 0.0f;
+            }
 #pragma unroll
-                for (int c4 = (i / 4) * 4; c4 < R; c4 += 4) {
-                    const f32x4 cv = *reinterpret_cast<const f32x4*>(&colbuf[p][rg * R + c4]);
+            for (int j = 0; j < 32; ++j) {
+                const float piv = bcast(d0[j], j);
+                const float inv = -1.0f / piv;
+                bad |= (piv == 0.0f) || !__builtin_isfinite(inv);
+                // scale pivot row j right of the diagonal; diagonal := -1/p
+                d0[j] = (lane > j) ? d0[j] * inv : ((lane == j) ? inv : d0[j]);
+                d1[j] = d1[j] * inv;
 #pragma unroll
-                    for (int e = 0; e < 4; ++e) {
-                        const int ii = c4 + e;
-                        if (ii > i || (ii == i && rg > rr)) d[ii] += rv * cv[e];
-                    }
-                    __builtin_amdgcn_sched_barrier(0);
+                for (int r = j + 1; r < 32; ++r) {
+                    const float s = bcast(d0[r], j);
+                    if (lane > j) d0[r] = d0[r] + d0[j] * s;
+                    d1[r] = d1[r] + d1[j] * s;
                 }
             }
-            // publish step gk+1 into the other buffers
-            const int nk = gk + 1;
-            if (nk < B) {
-                if (c == nk) {
-                    if (rg > rr) colbuf[1 - p][rg * R + i] = d[i];
 #pragma unroll
-                    for (int ii = i + 1; ii < R; ++ii) colbuf[1 - p][rg * R + ii] = d[ii];
-                }
-                // owner of row nk: rg == nk % RG, register nk / RG
-                if (rr + 1 < RG) {
-                    if (rg == rr + 1) rowbuf[1 - p][c] = d[i];
-                } else {
-                    if (i + 1 < R && rg == 0) rowbuf[1 - p][c] = d[(i + 1 < R) ? i + 1 : i];
-                }
+            for (int r = 0; r < 32; ++r) {
+                if (h0) Ds[(P0 + r) * LD + c0] = d0[r];
+                if (h1) Ds[(P0 + r) * LD + c1] = d1[r];
             }
         }
-    }
+        __syncthreads();
+        const int R0 = P0 + 32;  // first row / column after the panel
+        const int W = B - R0;    // rows below and columns right of the panel
+        if (W == 0) break;
+        // ---- 2. rows below: X[j][P0+i] += X[j][P0+k] * D[P0+k][P0+i], i > k
+        if (t < W) {
+            const int j = R0 + t;
+            float x[32];
 #pragma unroll
-    for (int i = 0; i < R; ++i) base[(int64_t)(rg + RG * i) * lda + c] = d[i];
+            for (int i = 0; i < 32; ++i) x[i] = Ds[j * LD + P0 + i];
+#pragma unroll
+            for (int k = 0; k < 32; ++k)
+#pragma unroll
+                for (int i = k + 1; i < 32; ++i)
+                    x[i] = x[i] + x[k] * Ds[(P0 + k) * LD + P0 + i];
+#pragma unroll
+            for (int i = 0; i < 32; ++i) Ds[j * LD + P0 + i] = x[i];
+        }
+        __syncthreads();
+        // ---- 3. trailing W x W: D[j][i] += D[P0+k][i] * D[j][P0+k], k = 0..31
+        const int tiles = (W / 4) * (W / 4);
+        for (int tt = t; tt < tiles; tt += NT) {
+            const int tj = R0 + (tt / (W / 4)) * 4, ti = R0 + (tt % (W / 4)) * 4;
+            float acc[4][4];
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) acc[a][b] = Ds[(tj + a) * LD + ti + b];
+            for (int k = 0; k < 32; ++k) {
+                float lv[4], tv[4];
+#pragma unroll
+                for (int a = 0; a < 4; ++a) lv[a] = Ds[(tj + a) * LD + P0 + k];
+#pragma unroll
+                for (int b = 0; b < 4; ++b) tv[b] = Ds[(P0 + k) * LD + ti + b];
+#pragma unroll
+                for (int a = 0; a < 4; ++a)
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) acc[a][b] = acc[a][b] + tv[b] * lv[a];
+            }
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) Ds[(tj + a) * LD + ti + b] = acc[a][b];
+        }
+        __syncthreads();
+    }
+
+    for (int e = t; e < B * B; e += NT) {
+        const int r = e / B, c = e % B;
+        base[(int64_t)r * lda + c] = Ds[r * LD + c];
+    }
     if (bad && status) atomicMax(status, 1);
 }
 
@@ -102,9 +138,9 @@ __global__ __launch_bounds__(NT) void lu_diag_kernel(float* __restrict__ A, int6
 hipError_t launch_lu_diag(float* A, int64_t lda, int64_t off, int B, int* status,
                           hipStream_t s) {
     switch (B) {
-        case 32: lu_diag_kernel<32, 256><<<1, 256, 0, s>>>(A, lda, off, status); break;
-        case 64: lu_diag_kernel<64, 256><<<1, 256, 0, s>>>(A, lda, off, status); break;
-        case 128: lu_diag_kernel<128, 512><<<1, 512, 0, s>>>(A, lda, off, status); break;
+        case 32: lu_diag_kernel<32><<<1, NT, 0, s>>>(A, lda, off, status); break;
+        case 64: lu_diag_kernel<64><<<1, NT, 0, s>>>(A, lda, off, status); break;
+        case 128: lu_diag_kernel<128><<<1, NT, 0, s>>>(A, lda, off, status); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -127,36 +127,67 @@ __global__ __launch_bounds__(64) void trsm_kernel(float* __restrict__ A, int64_t
     float* stripT = A + off * lda + off + B;          // top strip: rows off.., cols off+B..
     float* stripL = A + (off + B) * lda + off;        // left strip: rows off+B.., cols off..
 
-    for (int q = 0; q < NQ; ++q) {
-        f32x16 acc;
-        // ---- load v_q in accumulator layout (row i = acc_row, vector = lr)
+    // prefetch of the next block's right-hand sides (registers)
+    float vn[16];
+    f32x4 sn[4];
+    auto fetch_v = [&](int q) {
         if (top) {
 #pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int i = acc_row(r, lane);
-                acc[r] = stripT[(int64_t)(q * 32 + i) * lda + v0 + lr];
-            }
+            for (int r = 0; r < 16; ++r)
+                vn[r] = stripT[(int64_t)(q * 32 + acc_row(r, lane)) * lda + v0 + lr];
         } else {
-            // coalesced 32 x 32 tile (vector rows) through LDS
 #pragma unroll
             for (int s = 0; s < 4; ++s) {
                 const int vr = (lane >> 3) + 8 * s, c4 = (lane & 7) * 4;
-                const f32x4 v = *reinterpret_cast<const f32x4*>(
-                    stripL + (v0 + vr) * lda + q * 32 + c4);
-                Stg[vr][c4 + 0] = v[0];
-                Stg[vr][c4 + 1] = v[1];
-                Stg[vr][c4 + 2] = v[2];
-                Stg[vr][c4 + 3] = v[3];
+                sn[s] = *reinterpret_cast<const f32x4*>(stripL + (v0 + vr) * lda + q * 32 + c4);
+            }
+        }
+    };
+    fetch_v(0);
+
+    constexpr int PF = 8;  // coefficient prefetch depth (k-pairs)
+    for (int q = 0; q < NQ; ++q) {
+        f32x16 acc;
+        // ---- v_q in accumulator layout (row i = acc_row, vector = lr)
+        if (top) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[r] = vn[r];
+        } else {
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const int vr = (lane >> 3) + 8 * s, c4 = (lane & 7) * 4;
+                Stg[vr][c4 + 0] = sn[s][0];
+                Stg[vr][c4 + 1] = sn[s][1];
+                Stg[vr][c4 + 2] = sn[s][2];
+                Stg[vr][c4 + 3] = sn[s][3];
             }
             wave_lds_sync();
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[r] = Stg[lr][acc_row(r, lane)];
         }
-        // ---- z_q = v_q + C[q rows][0:32q] * Y[0:32q]
-        for (int kp = 0; kp < 16 * q; ++kp) {
-            const float a = Xc[(int64_t)(2 * kp + lh) * ldx + q * 32 + lr];
-            const float b = Y[2 * kp + lh][lr];
-            acc = mfma32(a, b, acc);
+        if (q + 1 < NQ) fetch_v(q + 1);
+        // inverse of the diagonal sub-system, loaded early
+        const float* Gq = G + q * 1024;
+        float g[16];
+#pragma unroll
+        for (int kp = 0; kp < 16; ++kp) g[kp] = Gq[(2 * kp + lh) * 32 + lr];
+        // ---- z_q = v_q + C[q rows][0:32q] * Y[0:32q]   (16q k-pairs)
+        const int cnt = 16 * q;
+        const float* xc = Xc + (int64_t)lh * ldx + q * 32 + lr;
+        float an[PF];
+#pragma unroll
+        for (int j = 0; j < PF; ++j) an[j] = (j < cnt) ? xc[(int64_t)(2 * j) * ldx] : 0.0f;
+        for (int kp = 0; kp < cnt; kp += PF) {
+            float ac[PF];
+#pragma unroll
+            for (int j = 0; j < PF; ++j) ac[j] = an[j];
+#pragma unroll
+            for (int j = 0; j < PF; ++j) {
+                const int kn = kp + PF + j;
+                if (kn < cnt) an[j] = xc[(int64_t)(2 * kn) * ldx];
+            }
+#pragma unroll
+            for (int j = 0; j < PF; ++j) acc = mfma32(ac[j], Y[2 * (kp + j) + lh][lr], acc);
         }
         // ---- y_q = G_q z_q
         wave_lds_sync();  // staging reads done before z overwrites the tile
@@ -164,13 +195,8 @@ __global__ __launch_bounds__(64) void trsm_kernel(float* __restrict__ A, int64_t
         for (int r = 0; r < 16; ++r) Z[acc_row(r, lane)][lr] = acc[r];
         wave_lds_sync();
         f32x16 y = {0};
-        const float* Gq = G + q * 1024;
 #pragma unroll
-        for (int kp = 0; kp < 16; ++kp) {
-            const float a = Gq[(2 * kp + lh) * 32 + lr];
-            const float b = Z[2 * kp + lh][lr];
-            y = mfma32(a, b, y);
-        }
+        for (int kp = 0; kp < 16; ++kp) y = mfma32(g[kp], Z[2 * kp + lh][lr], y);
 #pragma unroll
         for (int r = 0; r < 16; ++r) Y[q * 32 + acc_row(r, lane)][lr] = y[r];
         // ---- write back

@@ -50,7 +50,7 @@ def test_lu_block_bit_exact(ctx, B):
     assert np.array_equal(F.view(np.uint32), O.gefa_nopvt(A).view(np.uint32))
 
 
-@pytest.mark.parametrize("n,B", [(64, 32), (256, 64), (512, 128), (1024, 128), (768, 256),
+@pytest.mark.parametrize("n,B", [(64, 32), (320, 32), (256, 64), (576, 64), (512, 128), (1024, 128), (768, 256),
                                  (1024, 256), (1024, 512)])
 def test_gefa_matches_oracle_ref_generator(ctx, n, B):
     _say(f"gefa ref-generator n={n} B={B}")
