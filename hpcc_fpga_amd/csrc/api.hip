@@ -27,6 +27,7 @@ struct hpl_ctx {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     hipEvent_t evA = nullptr, evP = nullptr;  // next-panel region updated / panel ready
     bool lookahead = true;
+    int depth = 2;  // bulk trailing update every `depth` steps (K = depth * B)
     std::string err;
 
     // resident system
@@ -45,7 +46,7 @@ struct hpl_ctx {
 
     // GEFA workspaces
     uint32_t B = 0;             // block size of the resident factor (0: none)
-    float* dLt = nullptr;       // transposed left strip, 2 x (B x n), by step parity
+    float* dLt = nullptr;       // transposed left strips, 2 * depth slots of B x n
     size_t capLt = 0;
     float* dDT = nullptr;       // transposed diagonal block, 2 x (B x B), by step parity
     size_t capDT = 0;
@@ -154,7 +155,7 @@ hpl_status alloc_system(hpl_ctx* c, uint64_t n) {
 hpl_status alloc_factor_ws(hpl_ctx* c, uint32_t B) {
     const uint64_t n = c->n;
     const uint64_t N = n / B;
-    hpl_status st = ensure(c, c->dLt, c->capLt, 2 * (size_t)B * n * sizeof(float));
+    hpl_status st = ensure(c, c->dLt, c->capLt, 2 * (size_t)c->depth * B * n * sizeof(float));
     if (st) return st;
     st = ensure(c, c->dDT, c->capDT, 2 * (size_t)B * B * sizeof(float));
     if (st) return st;
@@ -220,8 +221,16 @@ hpl_status factor_diag(hpl_ctx* c, hipStream_t st, int64_t off, uint32_t B, floa
     return factor_diag(c, st, off + h, h, next);
 }
 
+// Slot of step k's transposed left strip: groups of `depth` consecutive steps
+// fill one half of the buffer in order, so a group's strips are contiguous
+// rows (k-major) and can feed one GEMM with K = depth * B.
+int64_t lt_slot(const hpl_ctx* c, int64_t k) {
+    return ((k / c->depth) % 2) * c->depth + (k % c->depth);
+}
+
 // Panel of step k: factor the diagonal block, prepare its inverses, solve the
-// top and left strips (the left one also into Lt[k % 2]).
+// top and left strips (the left one also, transposed, into its Lt slot with
+// absolute row indices).
 hpl_status panel(hpl_ctx* c, hipStream_t st, int64_t k, uint32_t B) {
     const int64_t n = (int64_t)c->n, lda = (int64_t)c->lda;
     const int64_t off = k * B;
@@ -231,49 +240,60 @@ hpl_status panel(hpl_ctx* c, hipStream_t st, int64_t k, uint32_t B) {
     float* DT = c->dDT + (k & 1) * (int64_t)B * B;
     HIPCHK(c, launch_prep(c->dA, lda, off, B, DT, inv, st));
     const int64_t m = n - off - B;
-    if (m > 0)
-        HIPCHK(c, launch_trsm(c->dA, lda, off, m, B, DT, inv, c->dLt + (k & 1) * (int64_t)B * n,
-                              n, st));
+    if (m > 0) {
+        float* Lt = c->dLt + lt_slot(c, k) * (int64_t)B * n + off + B;
+        HIPCHK(c, launch_trsm(c->dA, lda, off, m, B, DT, inv, Lt, n, st));
+    }
     return HPL_SUCCESS;
 }
 
 // Blocked right-looking schedule with depth-1 look-ahead (the pattern of the
-// reference's XRT backend, execution_xrt_pcie.hpp:328-415): the trailing
-// update of step k first updates the next panel's row and column blocks, the
-// panel of step k+1 then runs on stream s1 while stream s0 updates the rest.
+// reference's XRT backend, execution_xrt_pcie.hpp:328-415) and a delayed bulk
+// update.  At step k the row and column blocks of the next panel receive all
+// pending updates; the panel of step k+1 then runs on stream s1 while stream
+// s0 updates the rest of the trailing matrix -- but only every `depth` steps,
+// applying `depth` steps' updates in one GEMM with K = depth * B.  Every
+// element still receives each update exactly once, in step order.
 hpl_status gefa_schedule(hpl_ctx* c, uint32_t B) {
     const int64_t n = (int64_t)c->n, lda = (int64_t)c->lda;
     const int64_t N = n / B;
+    const int D = c->lookahead ? c->depth : 1;
     hpl_status r = panel(c, c->s0, 0, B);
     if (r) return r;
     for (int64_t k = 0; k + 1 < N; ++k) {
         const int64_t k1 = (k + 1) * B;  // first trailing row/column
         const int64_t m = n - k1;
-        const float* Lt = c->dLt + (k & 1) * (int64_t)B * n;
-        const float* T = c->dA + k * B * lda + k1;  // top strip of step k
+        const int64_t j = k % D;         // updates k-j .. k are pending
+        const int K = (int)((j + 1) * B);
+        // stacked operands of the pending steps (contiguous by construction)
+        const float* Lt = c->dLt + lt_slot(c, k - j) * (int64_t)B * n;
+        const float* T = c->dA + (k - j) * B * lda + k1;
         float* C = c->dA + k1 * lda + k1;
         if (!c->lookahead || m <= (int64_t)B) {
-            r = launch_gemm(c, c->s0, C, lda, Lt, n, T, lda, (int)m, (int)m, (int)B);
+            r = launch_gemm(c, c->s0, C, lda, Lt + k1, n, T, lda, (int)m, (int)m, K);
             if (r) return r;
             r = panel(c, c->s0, k + 1, B);
             if (r) return r;
             continue;
         }
         // next panel's region: row block k+1 and column block k+1
-        r = launch_gemm(c, c->s0, C, lda, Lt, n, T, lda, (int)B, (int)m, (int)B);
+        r = launch_gemm(c, c->s0, C, lda, Lt + k1, n, T, lda, (int)B, (int)m, K);
         if (r) return r;
-        r = launch_gemm(c, c->s0, C + B * lda, lda, Lt + B, n, T, lda, (int)(m - B), (int)B,
-                        (int)B);
+        r = launch_gemm(c, c->s0, C + B * lda, lda, Lt + k1 + B, n, T, lda, (int)(m - B), (int)B,
+                        K);
         if (r) return r;
         HIPCHK(c, hipEventRecord(c->evA, c->s0));
         HIPCHK(c, hipStreamWaitEvent(c->s1, c->evA, 0));
         r = panel(c, c->s1, k + 1, B);
         if (r) return r;
         HIPCHK(c, hipEventRecord(c->evP, c->s1));
-        // the rest of the trailing matrix
-        r = launch_gemm(c, c->s0, C + B * lda + B, lda, Lt + B, n, T + B, lda, (int)(m - B),
-                        (int)(m - B), (int)B);
-        if (r) return r;
+        // the rest of the trailing matrix, every D steps; pending updates left
+        // at the end are applied by the final step's full update (m == B)
+        if (j == D - 1) {
+            r = launch_gemm(c, c->s0, C + B * lda + B, lda, Lt + k1 + B, n, T + B, lda,
+                            (int)(m - B), (int)(m - B), K);
+            if (r) return r;
+        }
         HIPCHK(c, hipStreamWaitEvent(c->s0, c->evP, 0));
     }
     return HPL_SUCCESS;
@@ -325,6 +345,7 @@ hpl_status hpl_create(int device, hpl_ctx** out) {
     hpl_ctx* c = new hpl_ctx();
     c->device = device;
     if (const char* e = getenv("HPL_LOOKAHEAD")) c->lookahead = atoi(e) != 0;
+    if (const char* e = getenv("HPL_UPDATE_DEPTH")) c->depth = std::max(1, std::min(8, atoi(e)));
     if (hipStreamCreateWithFlags(&c->s0, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&c->s1, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->e0) != hipSuccess || hipEventCreate(&c->e1) != hipSuccess ||
