@@ -54,8 +54,15 @@ struct hpl_ctx {
     size_t capInv = 0;
     float* dInner = nullptr;    // recursion scratch for blocks > 128
     size_t capInner = 0;
+    void* dLpT = nullptr;       // 16-bit k-contiguous top-strip copies, 2*depth slots of n x B
+    void* dLpL = nullptr;       // 16-bit k-contiguous left-strip copies, same layout
+    size_t capLpT = 0, capLpL = 0;
+    int prec = 0;               // trailing-update precision of the current factorisation
     int* dStatus = nullptr;
     double* dWork = nullptr;    // residual partial sums
+    double* dR64 = nullptr;     // refinement residual (FP64)
+    double* dX64 = nullptr;     // refined solution (FP64)
+    size_t capIR = 0;
     size_t capWork = 0;
     double* dOut4 = nullptr;
     bool factored = false;
@@ -171,13 +178,33 @@ hpl_status alloc_factor_ws(hpl_ctx* c, uint32_t B) {
         st = ensure(c, c->dInner, c->capInner, need * sizeof(float));
         if (st) return st;
     }
+    if (c->prec != 0) {
+        const size_t lp = 2 * (size_t)c->depth * n * B * 2;  // bytes, 16-bit elements
+        st = ensure(c, c->dLpT, c->capLpT, lp);
+        if (st) return st;
+        st = ensure(c, c->dLpL, c->capLpL, lp);
+        if (st) return st;
+    }
     if (!c->dStatus) HIPCHK(c, hipMalloc(&c->dStatus, 64));
     return HPL_SUCCESS;
 }
 
-hpl_status launch_gemm(hpl_ctx* c, hipStream_t st, float* C, int64_t ldc, const float* Lt,
-                       int64_t ldl, const float* T, int64_t ldt, int M, int N, int K) {
+// Slot of step k's transposed left strip: groups of `depth` consecutive steps
+// fill one half of the buffer in order, so a group's strips are contiguous
+// rows (k-major) and can feed one GEMM with K = depth * B.
+int64_t lt_slot(const hpl_ctx* c, int64_t k) {
+    return ((k / c->depth) % 2) * c->depth + (k % c->depth);
+}
+
+// Trailing update C += L T over rows [r0, r0+M) and columns [c0, c0+N) of A,
+// applying the pending steps k0 .. k0 + K/B - 1 (their operands are stacked
+// contiguously by the slot layout).  FP32 operands: Lt slots (k-major) and the
+// top strips in A; 16-bit operands: the k-contiguous copies.
+hpl_status launch_update(hpl_ctx* c, hipStream_t st, int64_t k0, int64_t r0, int64_t c0, int M,
+                         int N, int K, uint32_t B) {
     if (M <= 0 || N <= 0) return HPL_SUCCESS;
+    const int64_t n = (int64_t)c->n, lda = (int64_t)c->lda;
+    float* C = c->dA + r0 * lda + c0;
     hipEvent_t a = nullptr, b = nullptr;
     if (c->prof) {
         if (c->ev_used == c->ev.size()) {
@@ -191,7 +218,18 @@ hpl_status launch_gemm(hpl_ctx* c, hipStream_t st, float* C, int64_t ldc, const 
         c->ev_used++;
         HIPCHK(c, hipEventRecord(a, st));
     }
-    HIPCHK(c, launch_gemm_update_f32(C, ldc, Lt, ldl, T, ldt, M, N, K, st));
+    const int64_t slot = lt_slot(c, k0);
+    if (c->prec == 0) {
+        const float* Lt = c->dLt + slot * (int64_t)B * n + r0;
+        const float* T = c->dA + k0 * B * lda + c0;
+        HIPCHK(c, launch_gemm_update_f32(C, lda, Lt, n, T, lda, M, N, K, st));
+    } else {
+        const size_t esz = 2;
+        const char* L = static_cast<const char*>(c->dLpL) + (slot * n * B + r0 * B) * esz;
+        const char* T = static_cast<const char*>(c->dLpT) + (slot * n * B + c0 * B) * esz;
+        HIPCHK(c, launch_gemm_update_lp(c->prec, C, lda, L, B, n * B, T, B, n * B, M, N, K,
+                                        (int)B, st));
+    }
     if (c->prof) {
         HIPCHK(c, hipEventRecord(b, st));
         c->prof_flops += 2.0 * M * (double)N * K;
@@ -221,13 +259,6 @@ hpl_status factor_diag(hpl_ctx* c, hipStream_t st, int64_t off, uint32_t B, floa
     return factor_diag(c, st, off + h, h, next);
 }
 
-// Slot of step k's transposed left strip: groups of `depth` consecutive steps
-// fill one half of the buffer in order, so a group's strips are contiguous
-// rows (k-major) and can feed one GEMM with K = depth * B.
-int64_t lt_slot(const hpl_ctx* c, int64_t k) {
-    return ((k / c->depth) % 2) * c->depth + (k % c->depth);
-}
-
 // Panel of step k: factor the diagonal block, prepare its inverses, solve the
 // top and left strips (the left one also, transposed, into its Lt slot with
 // absolute row indices).
@@ -241,8 +272,15 @@ hpl_status panel(hpl_ctx* c, hipStream_t st, int64_t k, uint32_t B) {
     HIPCHK(c, launch_prep(c->dA, lda, off, B, DT, inv, st));
     const int64_t m = n - off - B;
     if (m > 0) {
-        float* Lt = c->dLt + lt_slot(c, k) * (int64_t)B * n + off + B;
-        HIPCHK(c, launch_trsm(c->dA, lda, off, m, B, DT, inv, Lt, n, st));
+        const int64_t slot = lt_slot(c, k);
+        float* Lt = c->dLt + slot * (int64_t)B * n + off + B;
+        void* lpT = nullptr;
+        void* lpL = nullptr;
+        if (c->prec != 0) {
+            lpT = static_cast<char*>(c->dLpT) + (slot * n * B + (off + B) * B) * 2;
+            lpL = static_cast<char*>(c->dLpL) + (slot * n * B + (off + B) * B) * 2;
+        }
+        HIPCHK(c, launch_trsm(c->dA, lda, off, m, B, DT, inv, Lt, n, st, c->prec, lpT, lpL, B));
     }
     return HPL_SUCCESS;
 }
@@ -255,7 +293,7 @@ hpl_status panel(hpl_ctx* c, hipStream_t st, int64_t k, uint32_t B) {
 // applying `depth` steps' updates in one GEMM with K = depth * B.  Every
 // element still receives each update exactly once, in step order.
 hpl_status gefa_schedule(hpl_ctx* c, uint32_t B) {
-    const int64_t n = (int64_t)c->n, lda = (int64_t)c->lda;
+    const int64_t n = (int64_t)c->n;
     const int64_t N = n / B;
     const int D = c->lookahead ? c->depth : 1;
     hpl_status r = panel(c, c->s0, 0, B);
@@ -265,22 +303,18 @@ hpl_status gefa_schedule(hpl_ctx* c, uint32_t B) {
         const int64_t m = n - k1;
         const int64_t j = k % D;         // updates k-j .. k are pending
         const int K = (int)((j + 1) * B);
-        // stacked operands of the pending steps (contiguous by construction)
-        const float* Lt = c->dLt + lt_slot(c, k - j) * (int64_t)B * n;
-        const float* T = c->dA + (k - j) * B * lda + k1;
-        float* C = c->dA + k1 * lda + k1;
+        const int64_t k0 = k - j;
         if (!c->lookahead || m <= (int64_t)B) {
-            r = launch_gemm(c, c->s0, C, lda, Lt + k1, n, T, lda, (int)m, (int)m, K);
+            r = launch_update(c, c->s0, k0, k1, k1, (int)m, (int)m, K, B);
             if (r) return r;
             r = panel(c, c->s0, k + 1, B);
             if (r) return r;
             continue;
         }
         // next panel's region: row block k+1 and column block k+1
-        r = launch_gemm(c, c->s0, C, lda, Lt + k1, n, T, lda, (int)B, (int)m, K);
+        r = launch_update(c, c->s0, k0, k1, k1, (int)B, (int)m, K, B);
         if (r) return r;
-        r = launch_gemm(c, c->s0, C + B * lda, lda, Lt + k1 + B, n, T, lda, (int)(m - B), (int)B,
-                        K);
+        r = launch_update(c, c->s0, k0, k1 + B, k1, (int)(m - B), (int)B, K, B);
         if (r) return r;
         HIPCHK(c, hipEventRecord(c->evA, c->s0));
         HIPCHK(c, hipStreamWaitEvent(c->s1, c->evA, 0));
@@ -290,8 +324,7 @@ hpl_status gefa_schedule(hpl_ctx* c, uint32_t B) {
         // the rest of the trailing matrix, every D steps; pending updates left
         // at the end are applied by the final step's full update (m == B)
         if (j == D - 1) {
-            r = launch_gemm(c, c->s0, C + B * lda + B, lda, Lt + k1 + B, n, T + B, lda,
-                            (int)(m - B), (int)(m - B), K);
+            r = launch_update(c, c->s0, k0, k1 + B, k1 + B, (int)(m - B), (int)(m - B), K, B);
             if (r) return r;
         }
         HIPCHK(c, hipStreamWaitEvent(c->s0, c->evP, 0));
@@ -365,7 +398,8 @@ void hpl_destroy(hpl_ctx* c) {
     if (c->s1) (void)hipStreamSynchronize(c->s1);
     for (void* p : {(void*)c->dA, (void*)c->dA0, (void*)c->db, (void*)c->dx, (void*)c->db64,
                     (void*)c->dxt, (void*)c->dLt, (void*)c->dDT, (void*)c->dInv,
-                    (void*)c->dInner, (void*)c->dStatus, (void*)c->dWork, (void*)c->dOut4})
+                    (void*)c->dInner, (void*)c->dStatus, (void*)c->dWork, (void*)c->dOut4,
+                    c->dLpT, c->dLpL, (void*)c->dR64, (void*)c->dX64})
         if (p) (void)hipFree(p);
     for (auto& e : c->ev) {
         (void)hipEventDestroy(e.first);
@@ -455,10 +489,12 @@ hpl_status hpl_gefa_dev(hpl_ctx* c, uint32_t block, hpl_prec prec, hpl_timing* t
         return fail(c, HPL_ERR_INVALID_ARG,
                     "block size %u invalid for n=%llu (power of two in [32,512] dividing n)",
                     block, (unsigned long long)c->n);
-    if (prec != HPL_PREC_FP32)
-        return fail(c, HPL_ERR_UNSUPPORTED, "precision %d not available in this entry point",
-                    (int)prec);
+    if (prec != HPL_PREC_FP32 && prec != HPL_PREC_BF16 && prec != HPL_PREC_FP16)
+        return fail(c, HPL_ERR_UNSUPPORTED, "precision %d not available", (int)prec);
+    if (prec != HPL_PREC_FP32 && block % 64)
+        return fail(c, HPL_ERR_INVALID_ARG, "16-bit trailing updates need a block multiple of 64");
     HIPCHK(c, hipSetDevice(c->device));
+    c->prec = (int)prec;
     hpl_status st = alloc_factor_ws(c, block);
     if (st) return st;
     HIPCHK(c, hipMemsetAsync(c->dStatus, 0, sizeof(int), c->s0));
@@ -568,6 +604,98 @@ hpl_status hpl_residual(hpl_ctx* c, double out[4]) {
     HIPCHK(c, launch_residual_f32x(c->dA0, (int64_t)c->lda, n, c->dx, c->db, c->dWork, nullptr,
                                    c->dOut4, c->s0));
     HIPCHK(c, hipMemcpyAsync(out, c->dOut4, 4 * sizeof(double), hipMemcpyDeviceToHost, c->s0));
+    HIPCHK(c, hipStreamSynchronize(c->s0));
+    return HPL_SUCCESS;
+}
+
+hpl_status hpl_solve_ir(hpl_ctx* c, uint32_t block, hpl_prec prec, int max_iter, double tol,
+                        hpl_ir_result* out) {
+    if (!c || !out) return HPL_ERR_INVALID_ARG;
+    if (!c->have_orig) return fail(c, HPL_ERR_STATE, "hpl_solve_ir needs hpl_keep_original(1)");
+    if (max_iter < 0) return fail(c, HPL_ERR_INVALID_ARG, "max_iter < 0");
+    HIPCHK(c, hipSetDevice(c->device));
+    const int64_t n = (int64_t)c->n;
+    hpl_status st = ensure(c, c->dWork, c->capWork, 2 * (size_t)residual_chunks(n) * n * sizeof(double));
+    if (st) return st;
+    if (!c->dOut4) HIPCHK(c, hipMalloc(&c->dOut4, 4 * sizeof(double)));
+    if ((size_t)n * sizeof(double) > c->capIR) {
+        if (c->dR64) (void)hipFree(c->dR64);
+        if (c->dX64) (void)hipFree(c->dX64);
+        c->dR64 = c->dX64 = nullptr;
+        c->capIR = 0;
+        HIPCHK(c, hipMalloc(&c->dR64, n * sizeof(double)));
+        HIPCHK(c, hipMalloc(&c->dX64, n * sizeof(double)));
+        c->capIR = n * sizeof(double);
+    }
+    // fresh matrix from the kept original (not timed)
+    HIPCHK(c, hipMemcpyAsync(c->dA, c->dA0, n * c->lda * sizeof(float), hipMemcpyDeviceToDevice,
+                             c->s0));
+    hipEvent_t t0, t1;
+    HIPCHK(c, hipEventCreate(&t0));
+    HIPCHK(c, hipEventCreate(&t1));
+    HIPCHK(c, hipEventRecord(t0, c->s0));
+    hpl_timing tg{};
+    st = hpl_gefa_dev(c, block, prec, &tg);
+    if (st) {
+        (void)hipEventDestroy(t0);
+        (void)hipEventDestroy(t1);
+        return st;
+    }
+    const int B = (int)c->B;
+    const int64_t inv_stride = 3 * (B / 32) * 1024;
+    // x = F^-1 b in FP32, then promote
+    HIPCHK(c, launch_d2f_scaled(c->db64, c->dx, n, 1.0, c->s0));
+    HIPCHK(c, launch_gesl(c->dA, (int64_t)c->lda, n, B, c->dInv, inv_stride, c->dx, c->s0));
+    HIPCHK(c, launch_axpy_f2d(c->dX64, c->dx, n, 1.0, 1, c->s0));
+    int it = 0;
+    double berr = 0.0, h[4];
+    for (;; ++it) {
+        HIPCHK(c, launch_residual_f64x(c->dA0, (int64_t)c->lda, n, c->dX64, c->db64, c->dWork,
+                                       c->dR64, c->dOut4, true, c->s0));
+        HIPCHK(c, hipMemcpyAsync(h, c->dOut4, sizeof h, hipMemcpyDeviceToHost, c->s0));
+        HIPCHK(c, hipStreamSynchronize(c->s0));
+        berr = h[0] / (h[1] * h[2] + h[3]);
+        if (berr <= tol || it >= max_iter || !(h[0] > 0.0)) break;
+        // correction: solve F d = r / ||r|| in FP32, x += ||r|| d in FP64
+        HIPCHK(c, launch_d2f_scaled(c->dR64, c->dx, n, 1.0 / h[0], c->s0));
+        HIPCHK(c, launch_gesl(c->dA, (int64_t)c->lda, n, B, c->dInv, inv_stride, c->dx, c->s0));
+        HIPCHK(c, launch_axpy_f2d(c->dX64, c->dx, n, h[0], 0, c->s0));
+    }
+    HIPCHK(c, hipEventRecord(t1, c->s0));
+    HIPCHK(c, hipEventSynchronize(t1));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, t0, t1));
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    out->iterations = it;
+    out->converged = berr <= tol ? 1 : 0;
+    out->backward_error = berr;
+    out->residual_inf = h[0];
+    out->norm_a = h[1];
+    out->norm_x = h[2];
+    out->norm_b = h[3];
+    out->gefa_s = tg.gefa_s;
+    out->total_s = ms * 1e-3;
+    c->factored = true;
+    if (!out->converged)
+        return fail(c, HPL_ERR_NOT_CONVERGED,
+                    "refinement stopped after %d iterations at backward error %.3e (tol %.3e)", it,
+                    berr, tol);
+    return HPL_SUCCESS;
+}
+
+hpl_status hpl_download_solution64(hpl_ctx* c, double* x) {
+    if (!c || !x) return HPL_ERR_INVALID_ARG;
+    if (!c->dX64) return fail(c, HPL_ERR_STATE, "no refined solution");
+    HIPCHK(c, hipMemcpy(x, c->dX64, c->n * sizeof(double), hipMemcpyDeviceToHost));
+    return HPL_SUCCESS;
+}
+
+hpl_status hpl_upload_rhs64(hpl_ctx* c, const double* b) {
+    if (!c || !b) return HPL_ERR_INVALID_ARG;
+    if (!c->dA) return fail(c, HPL_ERR_STATE, "no resident system");
+    HIPCHK(c, hipMemcpy(c->db64, b, c->n * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(c, launch_d2f_scaled(c->db64, c->db, (int64_t)c->n, 1.0, c->s0));
     HIPCHK(c, hipStreamSynchronize(c->s0));
     return HPL_SUCCESS;
 }

@@ -1,0 +1,360 @@
+// Batched solves of many independent systems of uneven order in one
+// persistent launch (no reference counterpart: the reference solves one
+// matrix per run; each system here follows the same algorithm -- LU without
+// pivoting in the dgefa encoding, forward/back substitution as
+// LINPACK/src/host/linpack_data.cpp:166-229 -- plus FP64 refinement).
+//
+// Work distribution: a queue of system indices ordered by decreasing size;
+// every workgroup takes the next index with one atomic add and solves that
+// system alone, until the queue is empty (every workgroup reaches the exit
+// after at most nsys + grid claims).  Largest-first keeps the tail short.
+//
+// One system per workgroup, blocked right-looking LU with 32-wide steps:
+//   1. wave 0 factorises the 32 x 32 diagonal block in registers (readlane
+//      broadcasts, no barrier), bit-identical to gefa_ref_nopvt on the block;
+//   2. the top strip (columns right of it) and the left strip (rows below)
+//      get the reference's closed-form solves, one column / row per thread,
+//      and are converted to FP16 (or kept FP32) in LDS;
+//   3. the trailing matrix receives C += Left * Top on MFMA: 32 x 32 tiles,
+//      v_mfma_f32_32x32x16_f16 (FP16 inputs, FP32 accumulation) or
+//      v_mfma_f32_32x32x2_f32 (FP32 path).
+// Then substitution in FP32 and classic refinement with FP64 residuals
+// against the untouched FP32 matrix until ||r|| / (||M|| ||x|| + ||b||) <= tol.
+//
+// Storage of system s: rows of length npad (= n rounded up to 32), matrix at
+// a_off[s] (row-major, M = A^T as everywhere), vectors at v_off[s].  The pad
+// is an identity extension written by the generator, so it changes nothing.
+#include "common.hpp"
+#include "kernels.hpp"
+
+#include <type_traits>
+
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+
+namespace {
+
+__device__ __forceinline__ float bcast(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+template <int NT>
+__device__ __forceinline__ double block_max(double v, double* red) {
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[w] = v;
+    __syncthreads();
+    double r = red[0];
+    for (int i = 1; i < NT / 64; ++i) r = fmax(r, red[i]);
+    return r;
+}
+
+// Solve M x = rhs in place on x (FP32), factor in A (npad, dgefa encoding).
+template <int NT>
+__device__ void substitute(const float* __restrict__ A, int npad, float* __restrict__ x) {
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    // forward: x_e += sum_p A[p][e] x_p, block by block
+    for (int k0 = 0; k0 < npad; k0 += 32) {
+        if (wave == 0 && lane < 32) {
+            float xi = x[k0 + lane];
+            for (int p = 0; p < 32; ++p) {
+                const float xp = __shfl(xi, p);
+                if (lane > p) xi += xp * A[(int64_t)(k0 + p) * npad + k0 + lane];
+            }
+            x[k0 + lane] = xi;
+        }
+        __syncthreads();
+        for (int e = k0 + 32 + t; e < npad; e += NT) {
+            float a = 0.0f;
+#pragma unroll 8
+            for (int p = 0; p < 32; ++p) a = fmaf(A[(int64_t)(k0 + p) * npad + e], x[k0 + p], a);
+            x[e] += a;
+        }
+        __syncthreads();
+    }
+    // backward: x_k = x_k / u_kk (stored -1/u_kk), x_i -= x_k u_ik (i < k)
+    for (int k0 = npad - 32; k0 >= 0; k0 -= 32) {
+        if (wave == 0 && lane < 32) {
+            float xi = x[k0 + lane];
+            for (int p = 31; p >= 0; --p) {
+                float sc = __shfl(xi, p) * A[(int64_t)(k0 + p) * npad + k0 + p];
+                if (lane == p) xi = -sc;
+                if (lane < p) xi += sc * A[(int64_t)(k0 + p) * npad + k0 + lane];
+            }
+            x[k0 + lane] = xi;
+        }
+        __syncthreads();
+        for (int e = t; e < k0; e += NT) {
+            float a = 0.0f;
+#pragma unroll 8
+            for (int p = 0; p < 32; ++p) a = fmaf(A[(int64_t)(k0 + p) * npad + e], x[k0 + p], a);
+            x[e] -= a;
+        }
+        __syncthreads();
+    }
+}
+
+// LDS layout (dynamic): D[32][33] f32 | Left[MAXM][40] T | TopT[MAXM][40] T | red[16] f64
+template <int NT, bool F16>
+__global__ __launch_bounds__(NT) void batch_solve_kernel(
+    const BatchSystem* __restrict__ sys, const int* __restrict__ order, int nsys,
+    int* __restrict__ counter, float* __restrict__ A, const float* __restrict__ A0,
+    const double* __restrict__ b64, double* __restrict__ x64, float* __restrict__ xs,
+    double* __restrict__ rs, int max_iter, double tol_per_n, BatchResult* __restrict__ out,
+    int maxm) {
+    typedef typename std::conditional<F16, _Float16, float>::type T;
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+    constexpr int LDT = 40;  // padded row of the strip copies (elements)
+    float* D = reinterpret_cast<float*>(smem);
+    T* Left = reinterpret_cast<T*>(smem + 32 * 33 * 4);
+    T* TopT = Left + (size_t)maxm * LDT;
+    double* red = reinterpret_cast<double*>(TopT + (size_t)maxm * LDT);
+    __shared__ int claim;
+
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int lr = lane & 31, lh = lane >> 5;
+    for (;;) {
+        if (t == 0) claim = atomicAdd(counter, 1);
+        __syncthreads();
+        const int q = claim;
+        __syncthreads();
+        if (q >= nsys) break;
+        const int s = order[q];
+        const int npad = sys[s].npad;
+        float* a = A + sys[s].a_off;
+        bool bad = false;
+
+        // ------------------------------------------------------------ factor
+        for (int k0 = 0; k0 < npad; k0 += 32) {
+            if (wave == 0) {
+                // diagonal block: lane c < 32 holds column k0+c, rows 0..31
+                float d[32];
+#pragma unroll
+                for (int r = 0; r < 32; ++r)
+                    d[r] = lr == lane ? a[(int64_t)(k0 + r) * npad + k0 + lr] : 0.0f;
+#pragma unroll
+                for (int j = 0; j < 32; ++j) {
+                    const float piv = bcast(d[j], j);
+                    const float inv = -1.0f / piv;
+                    bad |= (piv == 0.0f) || !__builtin_isfinite(inv);
+                    d[j] = (lane > j) ? __fmul_rn(d[j], inv) : ((lane == j) ? inv : d[j]);
+#pragma unroll
+                    for (int r = j + 1; r < 32; ++r) {
+                        const float sv = bcast(d[r], j);
+                        if (lane > j) d[r] = __fadd_rn(d[r], __fmul_rn(d[j], sv));
+                    }
+                }
+                if (lane < 32) {
+#pragma unroll
+                    for (int r = 0; r < 32; ++r) {
+                        a[(int64_t)(k0 + r) * npad + k0 + lane] = d[r];
+                        D[r * 33 + lane] = d[r];
+                    }
+                }
+            }
+            __syncthreads();
+            const int r0 = k0 + 32, m = npad - r0;
+            if (m > 0) {
+                // top strip: columns c >= r0 of rows k0..k0+31 (closed form top)
+                for (int c = t; c < m; c += NT) {
+                    float x[32];
+#pragma unroll
+                    for (int i = 0; i < 32; ++i) x[i] = a[(int64_t)(k0 + i) * npad + r0 + c];
+#pragma unroll
+                    for (int kk = 0; kk < 32; ++kk) {
+                        x[kk] = __fmul_rn(x[kk], D[kk * 33 + kk]);
+#pragma unroll
+                        for (int j = kk + 1; j < 32; ++j)
+                            x[j] = __fadd_rn(x[j], __fmul_rn(x[kk], D[j * 33 + kk]));
+                    }
+#pragma unroll
+                    for (int i = 0; i < 32; ++i) {
+                        a[(int64_t)(k0 + i) * npad + r0 + c] = x[i];
+                        TopT[(size_t)c * LDT + i] = (T)x[i];
+                    }
+                }
+                // left strip: rows r >= r0, columns k0..k0+31 (closed form left)
+                for (int r = t; r < m; r += NT) {
+                    float x[32];
+                    float* row = a + (int64_t)(r0 + r) * npad + k0;
+#pragma unroll
+                    for (int i = 0; i < 32; i += 4) {
+                        const f32x4 v = *reinterpret_cast<const f32x4*>(row + i);
+                        x[i] = v[0]; x[i + 1] = v[1]; x[i + 2] = v[2]; x[i + 3] = v[3];
+                    }
+#pragma unroll
+                    for (int kk = 0; kk < 32; ++kk)
+#pragma unroll
+                        for (int i = kk + 1; i < 32; ++i)
+                            x[i] = __fadd_rn(x[i], __fmul_rn(x[kk], D[kk * 33 + i]));
+#pragma unroll
+                    for (int i = 0; i < 32; i += 4)
+                        *reinterpret_cast<f32x4*>(row + i) = f32x4{x[i], x[i + 1], x[i + 2], x[i + 3]};
+#pragma unroll
+                    for (int i = 0; i < 32; ++i) Left[(size_t)r * LDT + i] = (T)x[i];
+                }
+                __syncthreads();
+                // trailing update on MFMA, 32 x 32 tiles over the waves
+                const int tiles = m / 32;
+                for (int tt = wave; tt < tiles * tiles; tt += NT / 64) {
+                    const int ti = tt / tiles, tj = tt % tiles;
+                    float* cbase = a + (int64_t)(r0 + ti * 32) * npad + r0 + tj * 32;
+                    f32x16 acc;
+#pragma unroll
+                    for (int rr = 0; rr < 16; ++rr)
+                        acc[rr] = cbase[(int64_t)acc_row(rr, lane) * npad + lr];
+                    if constexpr (F16) {
+#pragma unroll
+                        for (int ks = 0; ks < 2; ++ks) {
+                            const f16x8 av = *reinterpret_cast<const f16x8*>(
+                                &Left[(size_t)(ti * 32 + lr) * LDT + ks * 16 + 8 * lh]);
+                            const f16x8 bv = *reinterpret_cast<const f16x8*>(
+                                &TopT[(size_t)(tj * 32 + lr) * LDT + ks * 16 + 8 * lh]);
+                            acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(av, bv, acc, 0, 0, 0);
+                        }
+                    } else {
+#pragma unroll
+                        for (int kp = 0; kp < 16; ++kp) {
+                            const float av = Left[(size_t)(ti * 32 + lr) * LDT + 2 * kp + lh];
+                            const float bv = TopT[(size_t)(tj * 32 + lr) * LDT + 2 * kp + lh];
+                            acc = mfma32(av, bv, acc);
+                        }
+                    }
+#pragma unroll
+                    for (int rr = 0; rr < 16; ++rr)
+                        cbase[(int64_t)acc_row(rr, lane) * npad + lr] = acc[rr];
+                }
+            }
+            __syncthreads();
+        }
+
+        // ------------------------------------------------- solve + refinement
+        const int64_t vo = sys[s].v_off;
+        const float* a0 = A0 + sys[s].a_off;
+        float* x32 = xs + vo;
+        double* x = x64 + vo;
+        double* r = rs + vo;
+        const double* b = b64 + vo;
+        for (int i = t; i < npad; i += NT) x32[i] = (float)b[i];
+        __syncthreads();
+        substitute<NT>(a, npad, x32);
+        for (int i = t; i < npad; i += NT) x[i] = (double)x32[i];
+        __syncthreads();
+        double anorm = 0.0, berr = 0.0;
+        int it = 0;
+        for (;; ++it) {
+            double rmax = 0.0, amax = 0.0, xmax = 0.0, bmax = 0.0;
+            for (int i = t; i < npad; i += NT) {
+                double acc = 0.0, sa = 0.0;
+                for (int j = 0; j < npad; ++j) {
+                    const double v = (double)a0[(int64_t)j * npad + i];
+                    acc = fma(v, x[j], acc);
+                    sa += fabs(v);
+                }
+                const double ri = b[i] - acc;
+                r[i] = ri;
+                rmax = fmax(rmax, fabs(ri));
+                amax = fmax(amax, sa);
+                xmax = fmax(xmax, fabs(x[i]));
+                bmax = fmax(bmax, fabs(b[i]));
+            }
+            rmax = block_max<NT>(rmax, red);
+            if (it == 0) anorm = block_max<NT>(amax, red);
+            xmax = block_max<NT>(xmax, red);
+            bmax = block_max<NT>(bmax, red);
+            berr = rmax / (anorm * xmax + bmax);
+            if (berr <= tol_per_n * sys[s].n || it >= max_iter || !(rmax > 0.0)) break;
+            for (int i = t; i < npad; i += NT) x32[i] = (float)(r[i] / rmax);
+            __syncthreads();
+            substitute<NT>(a, npad, x32);
+            for (int i = t; i < npad; i += NT) x[i] = fma(rmax, (double)x32[i], x[i]);
+            __syncthreads();
+        }
+        if (t == 0) {
+            out[s].iterations = it;
+            out[s].backward_error = berr;
+            out[s].status = bad ? 1 : 0;
+        }
+        __syncthreads();
+    }
+}
+
+// Generator: system s, row-major npad x npad; M(i,j) = U[-0.5,0.5) for
+// i, j < n (counter-based, key of (seed, s)), M(i,i) += n, identity on the
+// pad; b = M * 1 in FP64 on the first n entries, 0 on the pad.
+__global__ void batch_generate_kernel(const BatchSystem* __restrict__ sys, int nsys, uint64_t seed,
+                                      float* __restrict__ A, double* __restrict__ b64) {
+    const int s = blockIdx.x;
+    if (s >= nsys) return;
+    const int n = sys[s].n, npad = sys[s].npad;
+    float* a = A + sys[s].a_off;
+    const uint64_t key = hpl_stream_key(seed, 2 + (uint64_t)s);
+    for (int64_t e = threadIdx.x; e < (int64_t)npad * npad; e += blockDim.x) {
+        const int r = (int)(e / npad), c = (int)(e % npad);
+        float v;
+        if (r < n && c < n)
+            v = hpl_u24(key, (uint64_t)r * n + c) - 0.5f + (r == c ? (float)n : 0.0f);
+        else
+            v = (r == c) ? 1.0f : 0.0f;
+        a[e] = v;
+    }
+    __syncthreads();
+    double* b = b64 + sys[s].v_off;
+    for (int i = threadIdx.x; i < npad; i += blockDim.x) {
+        double acc = 0.0;
+        if (i < n)
+            for (int j = 0; j < n; ++j) acc += (double)a[(int64_t)j * npad + i];
+        b[i] = acc;
+    }
+}
+
+__global__ void reset_counter_kernel(int* c) {
+    if (threadIdx.x == 0) *c = 0;
+}
+
+}  // namespace
+
+hipError_t batch_prepare() {
+    const int lds = 160 * 1024;
+    const void* ks[] = {reinterpret_cast<const void*>(batch_solve_kernel<128, true>),
+                        reinterpret_cast<const void*>(batch_solve_kernel<128, false>),
+                        reinterpret_cast<const void*>(batch_solve_kernel<256, true>),
+                        reinterpret_cast<const void*>(batch_solve_kernel<256, false>)};
+    for (const void* k : ks) {
+        hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+size_t batch_lds_bytes(int maxm, bool f16) {
+    const size_t es = f16 ? 2 : 4;
+    return 32 * 33 * 4 + 2 * (size_t)maxm * 40 * es + 16 * sizeof(double);
+}
+
+hipError_t launch_batch_generate(const BatchSystem* sys, int nsys, uint64_t seed, float* A,
+                                 double* b64, hipStream_t s) {
+    batch_generate_kernel<<<nsys, 256, 0, s>>>(sys, nsys, seed, A, b64);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_solve(int threads, bool f16, int grid, const BatchSystem* sys,
+                              const int* order, int nsys, int* counter, float* A, const float* A0,
+                              const double* b64, double* x64, float* xs, double* rs, int max_iter,
+                              double tol_per_n, BatchResult* out, int maxm, hipStream_t s) {
+    const size_t lds = batch_lds_bytes(maxm, f16);
+    if (lds > 160 * 1024) return hipErrorInvalidValue;
+    reset_counter_kernel<<<1, 64, 0, s>>>(counter);
+#define HPL_BATCH(NTV, F)                                                                     \
+    batch_solve_kernel<NTV, F><<<grid, NTV, lds, s>>>(sys, order, nsys, counter, A, A0, b64, x64, \
+                                                     xs, rs, max_iter, tol_per_n, out, maxm)
+    if (threads == 256) {
+        if (f16) HPL_BATCH(256, true); else HPL_BATCH(256, false);
+    } else if (threads == 128) {
+        if (f16) HPL_BATCH(128, true); else HPL_BATCH(128, false);
+    } else {
+        return hipErrorInvalidValue;
+    }
+#undef HPL_BATCH
+    return hipGetLastError();
+}

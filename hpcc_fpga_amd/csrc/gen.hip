@@ -165,6 +165,34 @@ hipError_t launch_residual_f32x(const float* A, int64_t lda, int64_t n, const fl
     return launch_residual_t<float, float>(A, lda, n, x, b, work, r, out4, true, s);
 }
 
+namespace {
+__global__ void d2f_scaled_kernel(const double* __restrict__ r, float* __restrict__ x, int64_t n,
+                                  double scale) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        x[i] = (float)(r[i] * scale);
+}
+__global__ void axpy_f2d_kernel(double* __restrict__ x, const float* __restrict__ d, int64_t n,
+                                double scale, int assign) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        x[i] = assign ? scale * (double)d[i] : fma(scale, (double)d[i], x[i]);
+}
+}  // namespace
+
+hipError_t launch_d2f_scaled(const double* r, float* x, int64_t n, double scale, hipStream_t s) {
+    d2f_scaled_kernel<<<(unsigned)std::min<int64_t>((n + 255) / 256, 4096), 256, 0, s>>>(r, x, n,
+                                                                                        scale);
+    return hipGetLastError();
+}
+
+hipError_t launch_axpy_f2d(double* x, const float* d, int64_t n, double scale, int assign,
+                           hipStream_t s) {
+    axpy_f2d_kernel<<<(unsigned)std::min<int64_t>((n + 255) / 256, 4096), 256, 0, s>>>(
+        x, d, n, scale, assign);
+    return hipGetLastError();
+}
+
 hipError_t launch_residual_f64x(const float* A, int64_t lda, int64_t n, const double* x,
                                 const double* b, double* work, double* r, double* out4,
                                 bool with_norm, hipStream_t s) {

@@ -12,13 +12,21 @@ hipError_t launch_prep(const float* A, int64_t lda, int64_t off, int B, float* D
                        hipStream_t s);
 hipError_t launch_prep_all(const float* A, int64_t lda, int B, int nblocks, float* inv,
                            int64_t inv_stride, hipStream_t s);
+// lp_prec 0: FP32 only; 1/2: also write BF16/FP16 k-contiguous copies of the
+// top strip (lpT[c][k]) and left strip (lpL[r][k]), leading dimension ldlp.
 hipError_t launch_trsm(float* A, int64_t lda, int64_t off, int64_t m, int B, const float* DT,
-                       const float* inv, float* Lt, int64_t ldt, hipStream_t s);
+                       const float* inv, float* Lt, int64_t ldt, hipStream_t s, int lp_prec = 0,
+                       void* lpT = nullptr, void* lpL = nullptr, int64_t ldlp = 0);
 
 // trailing update (gemm.hip)
 hipError_t launch_gemm_update_f32(float* C, int64_t ldc, const float* Lt, int64_t ldl,
                                   const float* T, int64_t ldt, int M, int N, int K,
                                   hipStream_t s);
+
+// mixed-precision trailing update (gemm_lp.hip): prec 1 = BF16, 2 = FP16
+hipError_t launch_gemm_update_lp(int prec, float* C, int64_t ldc, const void* L, int64_t l_ld,
+                                 int64_t l_slot, const void* Tt, int64_t t_ld, int64_t t_slot,
+                                 int M, int N, int K, int Bblk, hipStream_t s);
 
 // substitution (gesl.hip)
 hipError_t launch_gesl(const float* A, int64_t lda, int64_t n, int B, const float* inv,
@@ -34,3 +42,29 @@ hipError_t launch_residual_f32x(const float* A, int64_t lda, int64_t n, const fl
 hipError_t launch_residual_f64x(const float* A, int64_t lda, int64_t n, const double* x,
                                 const double* b, double* work, double* r, double* out4,
                                 bool with_norm, hipStream_t s);
+
+// refinement vector helpers (gen.hip): x = (float)(r * scale); x = d*scale (+ x)
+hipError_t launch_d2f_scaled(const double* r, float* x, int64_t n, double scale, hipStream_t s);
+hipError_t launch_axpy_f2d(double* x, const float* d, int64_t n, double scale, int assign,
+                           hipStream_t s);
+
+// batched solves (batch.hip)
+struct BatchSystem {
+    int64_t a_off;  // matrix offset (elements) in the matrix pool
+    int64_t v_off;  // vector offset (elements) in the vector pools
+    int n;          // order
+    int npad;       // order rounded up to 32 (identity extension)
+};
+struct BatchResult {
+    int iterations;
+    int status;  // 1: zero or non-finite pivot
+    double backward_error;
+};
+hipError_t batch_prepare();  // once per process: allow 160 KiB dynamic LDS
+size_t batch_lds_bytes(int maxm, bool f16);
+hipError_t launch_batch_generate(const BatchSystem* sys, int nsys, uint64_t seed, float* A,
+                                 double* b64, hipStream_t s);
+hipError_t launch_batch_solve(int threads, bool f16, int grid, const BatchSystem* sys,
+                              const int* order, int nsys, int* counter, float* A, const float* A0,
+                              const double* b64, double* x64, float* xs, double* rs, int max_iter,
+                              double tol_per_n, BatchResult* out, int maxm, hipStream_t s);

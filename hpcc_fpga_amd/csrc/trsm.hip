@@ -22,6 +22,8 @@
 #include "common.hpp"
 #include "kernels.hpp"
 
+#include <type_traits>
+
 namespace {
 
 // ----------------------------------------------------------------------------
@@ -104,12 +106,16 @@ __global__ __launch_bounds__(64) void prep_kernel(const float* __restrict__ A, i
 // The solve.  Launch: one wave per workgroup, grid.x = ceil(m/32) vectors
 // groups for the top strip followed by the same count for the left strip.
 // ----------------------------------------------------------------------------
-template <int B>
+// LP: element type of the optional low-precision copies for the mixed
+// precision trailing update (float = none): lpT[c][k] (top strip, column c)
+// and lpL[r][k] (left strip, row r), k-contiguous with leading dimension ldlp.
+template <int B, typename LP>
 __global__ __launch_bounds__(64) void trsm_kernel(float* __restrict__ A, int64_t lda, int64_t off,
                                                   int64_t m, const float* __restrict__ DT,
                                                   const float* __restrict__ inv,
                                                   float* __restrict__ Lt, int64_t ldt,
-                                                  int ngroups_top) {
+                                                  int ngroups_top, LP* __restrict__ lpT,
+                                                  LP* __restrict__ lpL, int64_t ldlp) {
     constexpr int NQ = B / 32;
     __shared__ float Y[B][32];
     __shared__ float SZ[32][33];  // staging tile of the left strip, then z_q
@@ -199,6 +205,17 @@ __global__ __launch_bounds__(64) void trsm_kernel(float* __restrict__ A, int64_t
         for (int kp = 0; kp < 16; ++kp) y = mfma32(g[kp], Z[2 * kp + lh][lr], y);
 #pragma unroll
         for (int r = 0; r < 16; ++r) Y[q * 32 + acc_row(r, lane)][lr] = y[r];
+        // ---- low-precision copy: registers 4g..4g+3 are 4 consecutive k
+        if constexpr (!std::is_same<LP, float>::value) {
+            LP* dst = (top ? lpT : lpL) + (v0 + lr) * ldlp + q * 32 + 4 * lh;
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                struct alignas(8) P4 { LP v[4]; } p4;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) p4.v[e] = static_cast<LP>(y[4 * g + e]);
+                *reinterpret_cast<P4*>(dst + 8 * g) = p4;
+            }
+        }
         // ---- write back
         if (top) {
 #pragma unroll
@@ -242,18 +259,46 @@ hipError_t launch_prep_all(const float* A, int64_t lda, int B, int nblocks, floa
     return hipGetLastError();
 }
 
-hipError_t launch_trsm(float* A, int64_t lda, int64_t off, int64_t m, int B, const float* DT,
-                       const float* inv, float* Lt, int64_t ldt, hipStream_t s) {
-    if (m <= 0) return hipSuccess;
+template <typename LP>
+static hipError_t launch_trsm_t(float* A, int64_t lda, int64_t off, int64_t m, int B,
+                                const float* DT, const float* inv, float* Lt, int64_t ldt,
+                                LP* lpT, LP* lpL, int64_t ldlp, hipStream_t s) {
     const int groups = (int)(m / 32);
     dim3 grid(2 * groups);
+#define HPL_TRSM_CASE(BB)                                                                     \
+    case BB:                                                                                  \
+        trsm_kernel<BB, LP><<<grid, 64, 0, s>>>(A, lda, off, m, DT, inv, Lt, ldt, groups, lpT, \
+                                                lpL, ldlp);                                   \
+        break;
     switch (B) {
-        case 32: trsm_kernel<32><<<grid, 64, 0, s>>>(A, lda, off, m, DT, inv, Lt, ldt, groups); break;
-        case 64: trsm_kernel<64><<<grid, 64, 0, s>>>(A, lda, off, m, DT, inv, Lt, ldt, groups); break;
-        case 128: trsm_kernel<128><<<grid, 64, 0, s>>>(A, lda, off, m, DT, inv, Lt, ldt, groups); break;
-        case 256: trsm_kernel<256><<<grid, 64, 0, s>>>(A, lda, off, m, DT, inv, Lt, ldt, groups); break;
-        case 512: trsm_kernel<512><<<grid, 64, 0, s>>>(A, lda, off, m, DT, inv, Lt, ldt, groups); break;
+        HPL_TRSM_CASE(32)
+        HPL_TRSM_CASE(64)
+        HPL_TRSM_CASE(128)
+        HPL_TRSM_CASE(256)
+        HPL_TRSM_CASE(512)
         default: return hipErrorInvalidValue;
     }
+#undef HPL_TRSM_CASE
     return hipGetLastError();
+}
+
+hipError_t launch_trsm(float* A, int64_t lda, int64_t off, int64_t m, int B, const float* DT,
+                       const float* inv, float* Lt, int64_t ldt, hipStream_t s, int lp_prec,
+                       void* lpT, void* lpL, int64_t ldlp) {
+    if (m <= 0) return hipSuccess;
+    if (m % 32) return hipErrorInvalidValue;
+    switch (lp_prec) {
+        case 0:
+            return launch_trsm_t<float>(A, lda, off, m, B, DT, inv, Lt, ldt, nullptr, nullptr, 0, s);
+        case 1:
+            return launch_trsm_t<__bf16>(A, lda, off, m, B, DT, inv, Lt, ldt,
+                                         static_cast<__bf16*>(lpT), static_cast<__bf16*>(lpL),
+                                         ldlp, s);
+        case 2:
+            return launch_trsm_t<_Float16>(A, lda, off, m, B, DT, inv, Lt, ldt,
+                                           static_cast<_Float16*>(lpT),
+                                           static_cast<_Float16*>(lpL), ldlp, s);
+        default:
+            return hipErrorInvalidValue;
+    }
 }

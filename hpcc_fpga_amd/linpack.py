@@ -39,6 +39,18 @@ class HplError(RuntimeError):
         self.message = message
 
 
+class _IRResult(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("backward_error", C.c_double),
+                ("residual_inf", C.c_double), ("norm_a", C.c_double), ("norm_x", C.c_double),
+                ("norm_b", C.c_double), ("gefa_s", C.c_double), ("total_s", C.c_double)]
+
+
+class _BatchStats(C.Structure):
+    _fields_ = [("seconds", C.c_double), ("flops", C.c_double), ("gflops", C.c_double),
+                ("max_iterations", C.c_int), ("max_backward_error_per_n", C.c_double),
+                ("failed", C.c_int), ("zero_pivots", C.c_int)]
+
+
 class _Timing(C.Structure):
     _fields_ = [("gefa_s", C.c_double), ("gesl_s", C.c_double), ("h2d_s", C.c_double),
                 ("d2h_s", C.c_double)]
@@ -84,6 +96,18 @@ def load_library():
         "hpl_download": ([P, P, u64, P, T], i32),
         "hpl_residual": ([P, C.POINTER(C.c_double)], i32),
         "hpl_download_vectors": ([P, P, P, P], i32),
+        "hpl_solve_ir": ([P, u32, i32, i32, dbl, C.POINTER(_IRResult)], i32),
+        "hpl_download_solution64": ([P, P], i32),
+        "hpl_upload_rhs64": ([P, P], i32),
+        "hpl_batch_create": ([P, P, i32, C.POINTER(P)], i32),
+        "hpl_batch_destroy": ([P], None),
+        "hpl_batch_last_error": ([P], C.c_char_p),
+        "hpl_batch_generate": ([P, u64], i32),
+        "hpl_batch_set_system": ([P, i32, P, u64, P], i32),
+        "hpl_batch_autotune": ([P, i32, i32, dbl, P], i32),
+        "hpl_batch_solve": ([P, i32, i32, dbl, C.POINTER(_BatchStats)], i32),
+        "hpl_batch_get": ([P, i32, P, C.POINTER(i32), C.POINTER(dbl)], i32),
+        "hpl_batch_get_system": ([P, i32, P, P], i32),
         "hpl_set_profiling": ([P, i32], i32),
         "hpl_kernel_stats": ([P, C.POINTER(u64), C.POINTER(dbl), C.POINTER(dbl)], i32),
     }
@@ -212,6 +236,26 @@ class Context:
         self._check(self._L.hpl_residual(self._c, out))
         return tuple(out)
 
+    def solve_ir(self, block: int, prec: str = "bf16", max_iter: int = 50, tol: float = None,
+                 check: bool = True) -> dict:
+        """Mixed-precision factorisation + FP64 classic refinement (see hpl_solve_ir)."""
+        if tol is None:
+            tol = 1e-14 * self.n
+        res = _IRResult()
+        st = self._L.hpl_solve_ir(self._c, block, PRECISIONS[prec], max_iter, tol, C.byref(res))
+        if st != HPL_SUCCESS and (check or st != 7):
+            self._check(st)
+        return {f: getattr(res, f) for f, _ in _IRResult._fields_}
+
+    def download_solution64(self) -> np.ndarray:
+        x = np.empty(self.n, np.float64)
+        self._check(self._L.hpl_download_solution64(self._c, x.ctypes.data))
+        return x
+
+    def upload_rhs64(self, b: np.ndarray):
+        b = np.ascontiguousarray(b, np.float64)
+        self._check(self._L.hpl_upload_rhs64(self._c, b.ctypes.data))
+
     def set_profiling(self, enable: bool = True):
         self._check(self._L.hpl_set_profiling(self._c, int(enable)))
 
@@ -220,6 +264,68 @@ class Context:
         n, s, f = C.c_uint64(), C.c_double(), C.c_double()
         self._check(self._L.hpl_kernel_stats(self._c, C.byref(n), C.byref(s), C.byref(f)))
         return int(n.value), float(s.value), float(f.value)
+
+
+class Batch:
+    """Many independent systems of uneven order solved by one graph replay."""
+
+    def __init__(self, ctx: Context, orders):
+        self._L = ctx._L
+        self.ctx = ctx  # keeps the device context alive
+        self.n = np.ascontiguousarray(orders, np.int32)
+        self._b = C.c_void_p()
+        st = self._L.hpl_batch_create(ctx._c, self.n.ctypes.data, len(self.n), C.byref(self._b))
+        if st != HPL_SUCCESS:
+            raise HplError(st, self._L.hpl_status_string(st).decode())
+
+    def _check(self, st):
+        if st != HPL_SUCCESS:
+            raise HplError(st, self._L.hpl_batch_last_error(self._b).decode())
+
+    def close(self):
+        if self._b:
+            self._L.hpl_batch_destroy(self._b)
+            self._b = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def generate(self, seed: int = 7):
+        self._check(self._L.hpl_batch_generate(self._b, seed))
+
+    def set_system(self, s: int, a: np.ndarray, b: np.ndarray):
+        a = _f32(np.ascontiguousarray(a, np.float32))
+        b = np.ascontiguousarray(b, np.float64)
+        self._check(self._L.hpl_batch_set_system(self._b, s, a.ctypes.data, a.shape[1],
+                                                 b.ctypes.data))
+
+    def autotune(self, prec: str = "fp16", max_iter: int = 20, tol_per_n: float = 1e-14):
+        th = np.zeros(4, np.int32)
+        self._check(self._L.hpl_batch_autotune(self._b, PRECISIONS[prec], max_iter, tol_per_n,
+                                               th.ctypes.data))
+        return th
+
+    def solve(self, prec: str = "fp16", max_iter: int = 20, tol_per_n: float = 1e-14) -> dict:
+        st = _BatchStats()
+        self._check(self._L.hpl_batch_solve(self._b, PRECISIONS[prec], max_iter, tol_per_n,
+                                            C.byref(st)))
+        return {f: getattr(st, f) for f, _ in _BatchStats._fields_}
+
+    def get(self, s: int):
+        x = np.empty(int(self.n[s]), np.float64)
+        it, be = C.c_int(), C.c_double()
+        self._check(self._L.hpl_batch_get(self._b, s, x.ctypes.data, C.byref(it), C.byref(be)))
+        return x, it.value, be.value
+
+    def get_system(self, s: int):
+        npad = (int(self.n[s]) + 31) // 32 * 32
+        a = np.empty((npad, npad), np.float32)
+        b = np.empty(npad, np.float64)
+        self._check(self._L.hpl_batch_get_system(self._b, s, a.ctypes.data, b.ctypes.data))
+        return a, b
 
 
 def gefa(a: np.ndarray, block: int, device: int = 0) -> np.ndarray:

@@ -128,6 +128,71 @@ hpl_status hpl_download_vectors(hpl_ctx* ctx, float* b, double* b64, double* xtr
  *  out[0] = ||b - M x||inf, out[1] = ||M||inf, out[2] = ||x||inf, out[3] = ||b||inf. */
 hpl_status hpl_residual(hpl_ctx* ctx, double out[4]);
 
+/* ---- mixed precision + iterative refinement --------------------------------
+ * Classic (GMRES-free) refinement on the resident system, which must have
+ * been created with hpl_keep_original(1) and carries an FP64 right-hand side
+ * (hpl_generate, or hpl_upload_rhs64): the kept FP32 matrix is factorised with
+ * the trailing-update GEMM in `prec` (FP32 accumulation; panel LU and
+ * triangular solves in FP32), x0 = F^-1 b in FP32, then
+ *   r = b - M x (FP64 accumulation on the device), d = F^-1 (r/||r||) in FP32,
+ *   x += ||r|| d (FP64)
+ * until the normwise backward error ||r||/(||M|| ||x|| + ||b||) <= tol or
+ * max_iter corrections.  No reference counterpart: the reference's only
+ * refinement is the test-only FGMRES (LINPACK/src/host/gmres.c:53-232), whose
+ * HPL-AI residual (:181-198) this stopping test follows. */
+typedef struct hpl_ir_result {
+    int iterations;        /* corrections applied */
+    int converged;         /* backward error <= tol */
+    double backward_error; /* ||b - Mx||inf / (||M||inf ||x||inf + ||b||inf) */
+    double residual_inf, norm_a, norm_x, norm_b;
+    double gefa_s;         /* factorisation time */
+    double total_s;        /* factor + solve + refinement, device time */
+} hpl_ir_result;
+hpl_status hpl_solve_ir(hpl_ctx* ctx, uint32_t block, hpl_prec prec, int max_iter, double tol,
+                        hpl_ir_result* result);
+/* The refined FP64 solution of the last hpl_solve_ir. */
+hpl_status hpl_download_solution64(hpl_ctx* ctx, double* x);
+/* Replace the resident right-hand side by an FP64 vector (FP32 copy derived). */
+hpl_status hpl_upload_rhs64(hpl_ctx* ctx, const double* b);
+
+/* ---- batched solves of many small systems of uneven order ----------------
+ * No reference counterpart (the reference solves one matrix per run); every
+ * system is solved as above: LU without pivoting in the dgefa encoding, the
+ * trailing update in FP16 (or FP32) with FP32 accumulation, substitution in
+ * FP32, classic refinement with FP64 residuals until the normwise backward
+ * error <= tol_per_n * n.  Orders 1..992; one persistent work-queue launch per
+ * size bucket, all captured into one HIP graph. */
+typedef struct hpl_batch hpl_batch;
+typedef struct hpl_batch_stats {
+    double seconds;                  /* device time of the graph replay        */
+    double flops;                    /* sum of 2/3 n^3 + 2 n^2                 */
+    double gflops;                   /* flops / seconds / 1e9                  */
+    int max_iterations;              /* refinement corrections, worst system   */
+    double max_backward_error_per_n; /* max over systems of berr / n           */
+    int failed;                      /* systems above the tolerance            */
+    int zero_pivots;                 /* systems that met a zero pivot          */
+} hpl_batch_stats;
+hpl_status hpl_batch_create(hpl_ctx* ctx, const int32_t* n, int nsys, hpl_batch** batch);
+void hpl_batch_destroy(hpl_batch* batch);
+const char* hpl_batch_last_error(const hpl_batch* batch);
+/* Counter-based generator: system s gets M(i,j) = U[-0.5,0.5) (key of seed
+ * and s), M(i,i) += n, b = M * 1 in FP64. */
+hpl_status hpl_batch_generate(hpl_batch* batch, uint64_t seed);
+/* Host system s: row-major a (lda >= n, M = a^T), FP64 right-hand side. */
+hpl_status hpl_batch_set_system(hpl_batch* batch, int s, const float* a, uint64_t lda,
+                                const double* b);
+/* Time each candidate workgroup size per size bucket, keep the fastest;
+ * threads[4] (may be NULL) receives the choice per bucket (0: empty). */
+hpl_status hpl_batch_autotune(hpl_batch* batch, hpl_prec prec, int max_iter, double tol_per_n,
+                              int* threads);
+hpl_status hpl_batch_solve(hpl_batch* batch, hpl_prec prec, int max_iter, double tol_per_n,
+                           hpl_batch_stats* stats);
+/* Refined solution (n doubles), corrections and backward error of system s. */
+hpl_status hpl_batch_get(hpl_batch* batch, int s, double* x, int* iterations, double* berr);
+/* The padded matrix (npad x npad, npad = n rounded up to 32) and right-hand
+ * side (npad doubles) of system s as stored on the device. */
+hpl_status hpl_batch_get_system(hpl_batch* batch, int s, float* a, double* b);
+
 /* ---- kernel timing for roofline accounting -------------------------------
  * When enabled, every trailing-update GEMM launch is bracketed by HIP events
  * on the stream it runs on.  hpl_kernel_stats returns the number of launches

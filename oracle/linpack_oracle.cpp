@@ -144,6 +144,31 @@ void orc_generate_sized(float* A, float* b, double* b64, double* xtrue,
     }
 }
 
+// System s of a batch: the sized generator (mode 0) on stream 2 + s, stored
+// with leading dimension npad >= n and an identity extension on the pad;
+// b = M * 1 in double (0 on the pad).
+void orc_generate_batch_system(float* A, double* b64, int64_t n, int64_t npad, uint64_t seed,
+                               uint64_t s) {
+    const uint64_t key = stream_key(seed, 2 + s);
+    for (int64_t r = 0; r < npad; r++)
+        for (int64_t c = 0; c < npad; c++) {
+            float v;
+            if (r < n && c < n) {
+                v = u24(key, (uint64_t)(r * n + c)) - 0.5f;
+                if (r == c) v += (float)n;
+            } else {
+                v = (r == c) ? 1.0f : 0.0f;
+            }
+            A[r * npad + c] = v;
+        }
+    for (int64_t i = 0; i < npad; i++) {
+        double acc = 0.0;
+        if (i < n)
+            for (int64_t j = 0; j < n; j++) acc += (double)A[j * npad + i];
+        b64[i] = acc;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // A8: gefa_ref_nopvt, LINPACK/src/host/linpack_data.cpp:166-196.
 // ---------------------------------------------------------------------------

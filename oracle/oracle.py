@@ -46,6 +46,7 @@ def lib():
                                          C.c_uint64, C.c_int, C.c_double]
         L.orc_hash_uniform.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64]
         L.orc_hash_uniform.restype = C.c_float
+        L.orc_generate_batch_system.argtypes = [_f32p, _f64p, _i64, _i64, C.c_uint64, C.c_uint64]
         L.orc_gefa_nopvt.argtypes = [_f32p, _i64, _i64]
         L.orc_gefa_nopvt_omp.argtypes = [_f32p, _i64, _i64, C.c_int]
         L.orc_gesl_nopvt.argtypes = [_f32p, _f32p, _i64, _i64]
@@ -100,6 +101,24 @@ def generate_sized(n: int, seed: int = 42, mode: int = 0, d: float = 1.0):
     lib().orc_generate_sized(A, b.ctypes.data, b64.ctypes.data, xt.ctypes.data, n, n, seed,
                              mode, d)
     return A, b, b64, xt
+
+
+def generate_batch_system(n: int, seed: int, s: int):
+    """System s of a batch (padded to a multiple of 32 with an identity).
+
+    Returns (A_pad, b64_pad, npad)."""
+    npad = (n + 31) // 32 * 32
+    A = np.empty((npad, npad), np.float32)
+    b = np.empty(npad, np.float64)
+    lib().orc_generate_batch_system(A, b, n, npad, seed, s)
+    return A, b, npad
+
+
+def batch_orders(count: int, lo: int, hi: int, seed: int) -> np.ndarray:
+    """Orders n_i log-uniform in [lo, hi] from the counter-based generator
+    (stream 1 of the seed): n = round(exp(log lo + u * (log hi - log lo)))."""
+    u = np.array([hash_uniform(seed, 1, i) for i in range(count)], np.float64)
+    return np.rint(np.exp(np.log(lo) + u * (np.log(hi) - np.log(lo)))).astype(np.int32)
 
 
 def hash_uniform(seed: int, stream: int, k: int) -> float:
