@@ -27,7 +27,7 @@ struct hpl_ctx {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     hipEvent_t evA = nullptr, evP = nullptr;  // next-panel region updated / panel ready
     bool lookahead = true;
-    int depth = 2;  // bulk trailing update every `depth` steps (K = depth * B)
+    int depth = 1;  // bulk trailing update every `depth` steps (K = depth * B)
     std::string err;
 
     // resident system

@@ -19,7 +19,7 @@
 
 namespace {
 
-constexpr int kMaxBatchOrder = 992;  // LDS holds the two strips of the largest system
+constexpr int kMaxBatchOrder = 1024;  // LDS holds the two strips of the largest system
 constexpr int kBuckets = 4;
 
 struct Bucket {
@@ -92,7 +92,7 @@ void drop_graph(hpl_batch* b) {
 
 int grid_for(const hpl_batch* b, const Bucket& k, bool f16) {
     const size_t lds = batch_lds_bytes(k.maxm, f16);
-    int per_cu = (int)std::max<size_t>(1, (160 * 1024) / std::max<size_t>(lds, 1));
+    int per_cu = (int)std::max<size_t>(1, kBatchLdsLimit / std::max<size_t>(lds, 1));
     per_cu = std::min(per_cu, 2048 / k.threads);
     return std::max(1, std::min(k.count, b->cus * per_cu));
 }
@@ -140,6 +140,7 @@ hpl_status hpl_batch_create(hpl_ctx* ctx, const int32_t* n, int nsys, hpl_batch*
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, b->device) == hipSuccess) b->cus = prop.multiProcessorCount;
     if (batch_prepare() != hipSuccess) {
+        (void)hipGetLastError();
         delete b;
         return HPL_ERR_HIP;
     }
@@ -266,7 +267,7 @@ hpl_status hpl_batch_autotune(hpl_batch* b, hpl_prec prec, int max_iter, double 
     for (int i = 0; i < kBuckets; ++i) {
         Bucket& k = b->buckets[i];
         if (!k.count) continue;
-        if (!f16 && batch_lds_bytes(std::max(k.maxm, 32), false) > 160 * 1024) continue;
+        if (!f16 && batch_lds_bytes(std::max(k.maxm, 32), false) > kBatchLdsLimit) continue;
         float best = 1e30f;
         int best_t = k.threads;
         for (int t : cands) {
@@ -304,7 +305,7 @@ hpl_status hpl_batch_solve(hpl_batch* b, hpl_prec prec, int max_iter, double tol
     const bool f16 = prec == HPL_PREC_FP16;
     if (!f16)
         for (auto& k : b->buckets)
-            if (k.count && batch_lds_bytes(std::max(k.maxm, 32), false) > 160 * 1024)
+            if (k.count && batch_lds_bytes(std::max(k.maxm, 32), false) > kBatchLdsLimit)
                 return bfail(b, HPL_ERR_UNSUPPORTED, "FP32 batched updates need n <= 480");
     if (!b->exec || b->graph_prec != (int)prec || b->graph_iter != max_iter ||
         b->graph_tol != tol_per_n) {

@@ -94,7 +94,8 @@ __device__ void substitute(const float* __restrict__ A, int npad, float* __restr
     }
 }
 
-// LDS layout (dynamic): D[32][33] f32 | Left[MAXM][40] T | TopT[MAXM][40] T | red[16] f64
+// LDS layout (dynamic): D[32][33] f32 (the refinement's reduction scratch
+// aliases it) | Left[MAXM][40] T | TopT[MAXM][40] T
 template <int NT, bool F16>
 __global__ __launch_bounds__(NT) void batch_solve_kernel(
     const BatchSystem* __restrict__ sys, const int* __restrict__ order, int nsys,
@@ -108,15 +109,16 @@ __global__ __launch_bounds__(NT) void batch_solve_kernel(
     float* D = reinterpret_cast<float*>(smem);
     T* Left = reinterpret_cast<T*>(smem + 32 * 33 * 4);
     T* TopT = Left + (size_t)maxm * LDT;
-    double* red = reinterpret_cast<double*>(TopT + (size_t)maxm * LDT);
-    __shared__ int claim;
+    double* red = reinterpret_cast<double*>(smem);  // used after the factorisation only
+    int* claim = reinterpret_cast<int*>(D + 32);  // D[0][32]: padding, never a matrix element
 
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int lr = lane & 31, lh = lane >> 5;
     for (;;) {
-        if (t == 0) claim = atomicAdd(counter, 1);
         __syncthreads();
-        const int q = claim;
+        if (t == 0) *claim = atomicAdd(counter, 1);
+        __syncthreads();
+        const int q = *claim;
         __syncthreads();
         if (q >= nsys) break;
         const int s = order[q];
@@ -315,7 +317,7 @@ __global__ void reset_counter_kernel(int* c) {
 }  // namespace
 
 hipError_t batch_prepare() {
-    const int lds = 160 * 1024;
+    const int lds = (int)kBatchLdsLimit;
     const void* ks[] = {reinterpret_cast<const void*>(batch_solve_kernel<128, true>),
                         reinterpret_cast<const void*>(batch_solve_kernel<128, false>),
                         reinterpret_cast<const void*>(batch_solve_kernel<256, true>),
@@ -329,7 +331,7 @@ hipError_t batch_prepare() {
 
 size_t batch_lds_bytes(int maxm, bool f16) {
     const size_t es = f16 ? 2 : 4;
-    return 32 * 33 * 4 + 2 * (size_t)maxm * 40 * es + 16 * sizeof(double);
+    return 32 * 33 * 4 + 2 * (size_t)maxm * 40 * es;
 }
 
 hipError_t launch_batch_generate(const BatchSystem* sys, int nsys, uint64_t seed, float* A,
@@ -343,7 +345,7 @@ hipError_t launch_batch_solve(int threads, bool f16, int grid, const BatchSystem
                               const double* b64, double* x64, float* xs, double* rs, int max_iter,
                               double tol_per_n, BatchResult* out, int maxm, hipStream_t s) {
     const size_t lds = batch_lds_bytes(maxm, f16);
-    if (lds > 160 * 1024) return hipErrorInvalidValue;
+    if (lds > kBatchLdsLimit) return hipErrorInvalidValue;
     reset_counter_kernel<<<1, 64, 0, s>>>(counter);
 #define HPL_BATCH(NTV, F)                                                                     \
     batch_solve_kernel<NTV, F><<<grid, NTV, lds, s>>>(sys, order, nsys, counter, A, A0, b64, x64, \

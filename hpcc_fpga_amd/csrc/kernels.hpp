@@ -60,7 +60,8 @@ struct BatchResult {
     int status;  // 1: zero or non-finite pivot
     double backward_error;
 };
-hipError_t batch_prepare();  // once per process: allow 160 KiB dynamic LDS
+constexpr size_t kBatchLdsLimit = 160 * 1024;  // LDS per workgroup on gfx950
+hipError_t batch_prepare();  // allow kBatchLdsLimit bytes of dynamic LDS
 size_t batch_lds_bytes(int maxm, bool f16);
 hipError_t launch_batch_generate(const BatchSystem* sys, int nsys, uint64_t seed, float* A,
                                  double* b64, hipStream_t s);

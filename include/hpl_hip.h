@@ -160,7 +160,7 @@ hpl_status hpl_upload_rhs64(hpl_ctx* ctx, const double* b);
  * system is solved as above: LU without pivoting in the dgefa encoding, the
  * trailing update in FP16 (or FP32) with FP32 accumulation, substitution in
  * FP32, classic refinement with FP64 residuals until the normwise backward
- * error <= tol_per_n * n.  Orders 1..992; one persistent work-queue launch per
+ * error <= tol_per_n * n.  Orders 1..1024; one persistent work-queue launch per
  * size bucket, all captured into one HIP graph. */
 typedef struct hpl_batch hpl_batch;
 typedef struct hpl_batch_stats {
