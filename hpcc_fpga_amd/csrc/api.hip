@@ -27,7 +27,11 @@ struct hpl_ctx {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     hipEvent_t evA = nullptr, evP = nullptr;  // next-panel region updated / panel ready
     bool lookahead = true;
-    int depth = 1;  // bulk trailing update every `depth` steps (K = depth * B)
+    // Bulk trailing update every `depth` steps (K = depth * B).  The FP32
+    // update is MFMA-bound, so 1; the 16-bit update is bound by the FP32 C
+    // traffic (8 bytes per element per pass), which `depth` divides.
+    int depth = 1;
+    int depth_fp32 = 1, depth_lp = 8;
     std::string err;
 
     // resident system
@@ -162,7 +166,8 @@ hpl_status alloc_system(hpl_ctx* c, uint64_t n) {
 hpl_status alloc_factor_ws(hpl_ctx* c, uint32_t B) {
     const uint64_t n = c->n;
     const uint64_t N = n / B;
-    hpl_status st = ensure(c, c->dLt, c->capLt, 2 * (size_t)c->depth * B * n * sizeof(float));
+    hpl_status st = ensure(c, c->dLt, c->capLt,
+                           c->prec == 0 ? 2 * (size_t)c->depth * B * n * sizeof(float) : 4);
     if (st) return st;
     st = ensure(c, c->dDT, c->capDT, 2 * (size_t)B * B * sizeof(float));
     if (st) return st;
@@ -273,7 +278,8 @@ hpl_status panel(hpl_ctx* c, hipStream_t st, int64_t k, uint32_t B) {
     const int64_t m = n - off - B;
     if (m > 0) {
         const int64_t slot = lt_slot(c, k);
-        float* Lt = c->dLt + slot * (int64_t)B * n + off + B;
+        // the FP32 transposed copy feeds only the FP32 update
+        float* Lt = c->prec == 0 ? c->dLt + slot * (int64_t)B * n + off + B : nullptr;
         void* lpT = nullptr;
         void* lpL = nullptr;
         if (c->prec != 0) {
@@ -378,7 +384,8 @@ hpl_status hpl_create(int device, hpl_ctx** out) {
     hpl_ctx* c = new hpl_ctx();
     c->device = device;
     if (const char* e = getenv("HPL_LOOKAHEAD")) c->lookahead = atoi(e) != 0;
-    if (const char* e = getenv("HPL_UPDATE_DEPTH")) c->depth = std::max(1, std::min(8, atoi(e)));
+    if (const char* e = getenv("HPL_UPDATE_DEPTH"))
+        c->depth_fp32 = c->depth_lp = std::max(1, std::min(8, atoi(e)));
     if (hipStreamCreateWithFlags(&c->s0, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&c->s1, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->e0) != hipSuccess || hipEventCreate(&c->e1) != hipSuccess ||
@@ -495,6 +502,7 @@ hpl_status hpl_gefa_dev(hpl_ctx* c, uint32_t block, hpl_prec prec, hpl_timing* t
         return fail(c, HPL_ERR_INVALID_ARG, "16-bit trailing updates need a block multiple of 64");
     HIPCHK(c, hipSetDevice(c->device));
     c->prec = (int)prec;
+    c->depth = prec == HPL_PREC_FP32 ? c->depth_fp32 : c->depth_lp;
     hpl_status st = alloc_factor_ws(c, block);
     if (st) return st;
     HIPCHK(c, hipMemsetAsync(c->dStatus, 0, sizeof(int), c->s0));

@@ -222,9 +222,10 @@ __global__ __launch_bounds__(64) void trsm_kernel(float* __restrict__ A, int64_t
             for (int r = 0; r < 16; ++r)
                 stripT[(int64_t)(q * 32 + acc_row(r, lane)) * lda + v0 + lr] = y[r];
         } else {
+            if (Lt)
 #pragma unroll
-            for (int r = 0; r < 16; ++r)
-                Lt[(int64_t)(q * 32 + acc_row(r, lane)) * ldt + v0 + lr] = y[r];
+                for (int r = 0; r < 16; ++r)
+                    Lt[(int64_t)(q * 32 + acc_row(r, lane)) * ldt + v0 + lr] = y[r];
             wave_lds_sync();
 #pragma unroll
             for (int s = 0; s < 4; ++s) {
