@@ -52,6 +52,78 @@ def cpu_baseline(n=5120, seed=42):
     }
 
 
+def mixed_precision_section(ctx, n, B):
+    """Mixed precision + FP64 refinement (BF16 trailing update, FP32 panel and
+    solves, FP64 residual/correction) on A_ii = d * sum_j |A_ij|, x_true in
+    U[-1,1), against the plain FP32 factorisation of the same matrix."""
+    import numpy as np
+
+    flops = 2.0 / 3.0 * n ** 3 + 2.0 * n * n
+    out = {"workload": f"n={n}, block {B}, diagonal d*sum|A_ij|, b = A x_true (FP64)",
+           "tolerance": f"backward error <= 1e-14*n = {1e-14 * n:.3e}", "runs": []}
+    ctx.keep_original(True)
+    for d in (1.01, 1.1, 2.0):
+        ctx.generate(n, 1, 42, d)
+        _, _, xt = ctx.download_vectors()
+        row = {"d": d}
+        for prec in ("fp32", "bf16"):
+            ctx.solve_ir(B, prec, max_iter=30, check=False)  # warm-up of this shape
+            r = ctx.solve_ir(B, prec, max_iter=30, check=False)
+            x = ctx.download_solution64()
+            fwd = float(np.max(np.abs(x - xt)) / np.max(np.abs(xt)))
+            row[prec] = {"gflops": round(flops / r["total_s"] / 1e9, 1),
+                         "total_ms": round(r["total_s"] * 1e3, 2),
+                         "gefa_ms": round(r["gefa_s"] * 1e3, 2), "iterations": r["iterations"],
+                         "backward_error": r["backward_error"], "forward_error": fwd,
+                         "converged": bool(r["converged"])}
+        row["speedup_vs_fp32"] = round(row["bf16"]["gflops"] / row["fp32"]["gflops"], 2)
+        out["runs"].append(row)
+        print(f"[bench] mixed d={d}: bf16 {row['bf16']['gflops']} GFLOP/s, fp32 "
+              f"{row['fp32']['gflops']} GFLOP/s", file=sys.stderr, flush=True)
+    return out
+
+
+def batched_section(ctx, count=4096, seed=7):
+    """4096 systems, n log-uniform in [32, 1024], FP16 trailing update + FP64
+    refinement, one HIP-graph replay per solve; workgroup sizes autotuned."""
+    import numpy as np
+
+    from hpcc_fpga_amd.linpack import Batch
+    from hpcc_fpga_amd import linpack as _lp  # noqa: F401
+
+    u = np.array([_hash_uniform(seed, 1, i) for i in range(count)])
+    orders = np.rint(np.exp(np.log(32) + u * (np.log(1024) - np.log(32)))).astype(np.int32)
+    bt = Batch(ctx, orders)
+    bt.generate(seed)
+    threads = bt.autotune("fp16")
+    times, st = [], None
+    for _ in range(5):
+        st = bt.solve("fp16")
+        times.append(st["seconds"])
+    bt.close()
+    t = float(np.median(times))
+    print(f"[bench] batched: {st['flops'] / t / 1e9:.1f} GFLOP/s", file=sys.stderr, flush=True)
+    return {"workload": f"{count} systems, n log-uniform [32,1024] (seed {seed}), nrhs=1",
+            "gflops": round(st["flops"] / t / 1e9, 1), "median_replay_ms": round(t * 1e3, 3),
+            "max_iterations": st["max_iterations"],
+            "max_backward_error_per_n": st["max_backward_error_per_n"],
+            "failed": st["failed"], "autotuned_threads_per_bucket": [int(x) for x in threads]}
+
+
+def _hash_uniform(seed, stream, k):
+    """The counter-based uniform of the generators (splitmix64 finaliser)."""
+    M = (1 << 64) - 1
+
+    def mix(z):
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
+        return z ^ (z >> 31)
+
+    key = mix(seed ^ (stream << 56))
+    z = mix((key + (k + 1) * 0x9E3779B97F4A7C15) & M)
+    return (z >> 40) / 16777216.0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -61,6 +133,9 @@ def main():
     ap.add_argument("--block", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=5120)
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the mixed-precision and batched sections")
+    ap.add_argument("--mixed-n", type=int, default=65536)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -115,6 +190,11 @@ def main():
     value = flops * len(tot) / float(np.sum(tot)) / 1e9
     achieved = gemm_flops / gemm_s / 1e12 if gemm_s > 0 else 0.0
 
+    extra = {}
+    if not args.no_extra:
+        extra["mixed_precision"] = mixed_precision_section(ctx, args.mixed_n, B)
+        extra["batched"] = batched_section(ctx)
+
     cpu = None
     if not args.no_cpu_baseline:
         print(f"[bench] cpu baseline n={args.cpu_n} ...", file=sys.stderr, flush=True)
@@ -151,6 +231,7 @@ def main():
         "cpu_baseline": cpu,
         "device": dev,
     }
+    line.update(extra)
     ctx.close()
     print(json.dumps(line), flush=True)
 

@@ -4,6 +4,7 @@ libhpl_hip.so (built from hpcc_fpga_amd/csrc) implements the C ABI declared in
 include/hpl_hip.h; hpcc_fpga_amd.linpack binds it with ctypes, and the C++
 host under hpcc_fpga_amd/host is the drop-in LINPACK benchmark executable.
 """
-from .linpack import Context, HplError, Timing, gefa, gesl, load_library  # noqa: F401
+from .linpack import (Batch, Context, HplError, Server, Timing, gefa, gesl,  # noqa: F401
+                      load_library)
 
-__all__ = ["Context", "HplError", "Timing", "gefa", "gesl", "load_library"]
+__all__ = ["Batch", "Context", "HplError", "Server", "Timing", "gefa", "gesl", "load_library"]

@@ -18,69 +18,10 @@
 #include <vector>
 
 #include "../../include/hpl_hip.h"
+#include "ctx.hpp"
 #include "kernels.hpp"
 
-struct hpl_ctx {
-    int device = 0;
-    hipStream_t s0 = nullptr;   // trailing updates (and everything else)
-    hipStream_t s1 = nullptr;   // look-ahead panel factorisation
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    hipEvent_t evA = nullptr, evP = nullptr;  // next-panel region updated / panel ready
-    bool lookahead = true;
-    // Bulk trailing update every `depth` steps (K = depth * B).  The FP32
-    // update is MFMA-bound, so 1; the 16-bit update is bound by the FP32 C
-    // traffic (8 bytes per element per pass), which `depth` divides.
-    int depth = 1;
-    int depth_fp32 = 1, depth_lp = 8;
-    std::string err;
-
-    // resident system
-    uint64_t n = 0, lda = 0;
-    float* dA = nullptr;
-    size_t capA = 0;
-    float* dA0 = nullptr;  // untouched copy (keep_original)
-    size_t capA0 = 0;
-    bool keep_orig = false;
-    bool have_orig = false;
-    float* db = nullptr;
-    float* dx = nullptr;
-    double* db64 = nullptr;
-    double* dxt = nullptr;
-    size_t capVec = 0;
-
-    // GEFA workspaces
-    uint32_t B = 0;             // block size of the resident factor (0: none)
-    float* dLt = nullptr;       // transposed left strips, 2 * depth slots of B x n
-    size_t capLt = 0;
-    float* dDT = nullptr;       // transposed diagonal block, 2 x (B x B), by step parity
-    size_t capDT = 0;
-    float* dInv = nullptr;      // per block: 3 * (B/32) inverses of 32 x 32
-    size_t capInv = 0;
-    float* dInner = nullptr;    // recursion scratch for blocks > 128
-    size_t capInner = 0;
-    void* dLpT = nullptr;       // 16-bit k-contiguous top-strip copies, 2*depth slots of n x B
-    void* dLpL = nullptr;       // 16-bit k-contiguous left-strip copies, same layout
-    size_t capLpT = 0, capLpL = 0;
-    int prec = 0;               // trailing-update precision of the current factorisation
-    int* dStatus = nullptr;
-    double* dWork = nullptr;    // residual partial sums
-    double* dR64 = nullptr;     // refinement residual (FP64)
-    double* dX64 = nullptr;     // refined solution (FP64)
-    size_t capIR = 0;
-    size_t capWork = 0;
-    double* dOut4 = nullptr;
-    bool factored = false;
-
-    // roofline accounting of the trailing update
-    bool prof = false;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
-    size_t ev_used = 0;
-    double prof_flops = 0.0;
-};
-
-namespace {
-
-constexpr int kMaxLeafLU = 128;
+namespace hplx {
 
 hpl_status fail(hpl_ctx* c, hpl_status st, const char* fmt, ...) {
     if (c) {
@@ -94,6 +35,14 @@ hpl_status fail(hpl_ctx* c, hpl_status st, const char* fmt, ...) {
     return st;
 }
 
+}  // namespace hplx
+
+namespace {
+
+using hplx::fail;
+
+constexpr int kMaxLeafLU = 128;
+
 #define HIPCHK(ctx, expr)                                                                 \
     do {                                                                                  \
         hipError_t e_ = (expr);                                                           \
@@ -106,6 +55,11 @@ hpl_status fail(hpl_ctx* c, hpl_status st, const char* fmt, ...) {
 template <typename T>
 hpl_status ensure(hpl_ctx* c, T*& p, size_t& cap, size_t bytes) {
     if (bytes <= cap && p) return HPL_SUCCESS;
+    if (c->external) {
+        if (bytes == 0) return HPL_SUCCESS;
+        return fail(c, HPL_ERR_BUDGET, "workspace of %zu bytes exceeds the reserved %zu", bytes,
+                    cap);
+    }
     if (p) {
         (void)hipFree(p);
         p = nullptr;
@@ -128,11 +82,7 @@ bool valid_block(uint64_t n, uint32_t B) {
     return n > 0 && n % B == 0;
 }
 
-uint64_t device_lda(uint64_t n) {
-    uint64_t lda = (n + 63) / 64 * 64;
-    if (const char* e = getenv("HPL_LDA_PAD")) lda += (uint64_t)atoll(e);
-    return lda;
-}
+using hplx::device_lda;
 
 hpl_status alloc_system(hpl_ctx* c, uint64_t n) {
     const uint64_t lda = device_lda(n);
@@ -163,35 +113,27 @@ hpl_status alloc_system(hpl_ctx* c, uint64_t n) {
     return HPL_SUCCESS;
 }
 
+size_t lp_elem_bytes(int prec) { return prec == HPL_PREC_FP8 ? 1 : 2; }
+
 hpl_status alloc_factor_ws(hpl_ctx* c, uint32_t B) {
-    const uint64_t n = c->n;
-    const uint64_t N = n / B;
-    hpl_status st = ensure(c, c->dLt, c->capLt,
-                           c->prec == 0 ? 2 * (size_t)c->depth * B * n * sizeof(float) : 4);
+    const hplx::WsPlan w = hplx::plan_factor_ws(c->n, B, c->prec, c->depth);
+    hpl_status st = ensure(c, c->dLt, c->capLt, w.lt);
     if (st) return st;
-    st = ensure(c, c->dDT, c->capDT, 2 * (size_t)B * B * sizeof(float));
+    st = ensure(c, c->dDT, c->capDT, w.dt);
     if (st) return st;
-    const size_t inv_stride = 3 * (B / 32) * 1024;
-    st = ensure(c, c->dInv, c->capInv, N * inv_stride * sizeof(float));
+    st = ensure(c, c->dInv, c->capInv, w.inv);
     if (st) return st;
-    if (B > kMaxLeafLU) {
-        // recursion scratch per level h = B/2, B/4, .., 128: Lt and DT (h x h
-        // each) and the 3 h/32 inverses of 32 x 32
-        size_t need = 0;
-        for (uint32_t h = B / 2; h >= (uint32_t)kMaxLeafLU; h /= 2)
-            need += 2 * (size_t)h * h + 3 * (h / 32) * 1024;
-        st = ensure(c, c->dInner, c->capInner, need * sizeof(float));
-        if (st) return st;
-    }
-    if (c->prec != 0) {
-        const size_t lp = 2 * (size_t)c->depth * n * B * 2;  // bytes, 16-bit elements
-        st = ensure(c, c->dLpT, c->capLpT, lp);
-        if (st) return st;
-        st = ensure(c, c->dLpL, c->capLpL, lp);
-        if (st) return st;
-    }
-    if (!c->dStatus) HIPCHK(c, hipMalloc(&c->dStatus, 64));
-    return HPL_SUCCESS;
+    st = ensure(c, c->dInner, c->capInner, w.inner);
+    if (st) return st;
+    st = ensure(c, c->dLpT, c->capLpT, w.lp);
+    if (st) return st;
+    st = ensure(c, c->dLpL, c->capLpL, w.lp);
+    if (st) return st;
+    st = ensure(c, c->dScT, c->capScT, w.sc);
+    if (st) return st;
+    st = ensure(c, c->dScL, c->capScL, w.sc);
+    if (st) return st;
+    return ensure(c, c->dStatus, c->capStatus, w.status);
 }
 
 // Slot of step k's transposed left strip: groups of `depth` consecutive steps
@@ -229,11 +171,19 @@ hpl_status launch_update(hpl_ctx* c, hipStream_t st, int64_t k0, int64_t r0, int
         const float* T = c->dA + k0 * B * lda + c0;
         HIPCHK(c, launch_gemm_update_f32(C, lda, Lt, n, T, lda, M, N, K, st));
     } else {
-        const size_t esz = 2;
+        const size_t esz = lp_elem_bytes(c->prec);
         const char* L = static_cast<const char*>(c->dLpL) + (slot * n * B + r0 * B) * esz;
         const char* T = static_cast<const char*>(c->dLpT) + (slot * n * B + c0 * B) * esz;
-        HIPCHK(c, launch_gemm_update_lp(c->prec, C, lda, L, B, n * B, T, B, n * B, M, N, K,
-                                        (int)B, st));
+        if (c->prec == HPL_PREC_FP8) {
+            const int64_t ns = n / 32;
+            HIPCHK(c, launch_gemm_update_fp8(C, lda, L, B, n * B, T, B, n * B,
+                                             c->dScL + slot * ns + r0 / 32,
+                                             c->dScT + slot * ns + c0 / 32, ns, M, N, K, (int)B,
+                                             st));
+        } else {
+            HIPCHK(c, launch_gemm_update_lp(c->prec, C, lda, L, B, n * B, T, B, n * B, M, N, K,
+                                            (int)B, st));
+        }
     }
     if (c->prof) {
         HIPCHK(c, hipEventRecord(b, st));
@@ -282,11 +232,19 @@ hpl_status panel(hpl_ctx* c, hipStream_t st, int64_t k, uint32_t B) {
         float* Lt = c->prec == 0 ? c->dLt + slot * (int64_t)B * n + off + B : nullptr;
         void* lpT = nullptr;
         void* lpL = nullptr;
+        float* scT = nullptr;
+        float* scL = nullptr;
         if (c->prec != 0) {
-            lpT = static_cast<char*>(c->dLpT) + (slot * n * B + (off + B) * B) * 2;
-            lpL = static_cast<char*>(c->dLpL) + (slot * n * B + (off + B) * B) * 2;
+            const size_t esz = lp_elem_bytes(c->prec);
+            lpT = static_cast<char*>(c->dLpT) + (slot * n * B + (off + B) * B) * esz;
+            lpL = static_cast<char*>(c->dLpL) + (slot * n * B + (off + B) * B) * esz;
         }
-        HIPCHK(c, launch_trsm(c->dA, lda, off, m, B, DT, inv, Lt, n, st, c->prec, lpT, lpL, B));
+        if (c->prec == HPL_PREC_FP8) {
+            scT = c->dScT + slot * (n / 32) + (off + B) / 32;
+            scL = c->dScL + slot * (n / 32) + (off + B) / 32;
+        }
+        HIPCHK(c, launch_trsm(c->dA, lda, off, m, B, DT, inv, Lt, n, st, c->prec, lpT, lpL, B, scT,
+                              scL));
     }
     return HPL_SUCCESS;
 }
@@ -340,7 +298,8 @@ hpl_status gefa_schedule(hpl_ctx* c, uint32_t B) {
 
 hpl_status check_status(hpl_ctx* c) {
     int h = 0;
-    HIPCHK(c, hipMemcpy(&h, c->dStatus, sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpyAsync(&h, c->dStatus, sizeof(int), hipMemcpyDeviceToHost, c->s0));
+    HIPCHK(c, hipStreamSynchronize(c->s0));
     if (h) return fail(c, HPL_ERR_ZERO_PIVOT, "zero or non-finite pivot met during GEFA");
     return HPL_SUCCESS;
 }
@@ -352,6 +311,126 @@ double elapsed_s(hpl_ctx* c) {
 }
 
 }  // namespace
+
+namespace hplx {
+
+uint64_t device_lda(uint64_t n) {
+    uint64_t lda = (n + 63) / 64 * 64;
+    if (const char* e = getenv("HPL_LDA_PAD")) lda += (uint64_t)atoll(e);
+    return lda;
+}
+
+WsPlan plan_factor_ws(uint64_t n, uint32_t B, int prec, int depth) {
+    WsPlan w{};
+    const uint64_t N = n / B;
+    w.lt = prec == HPL_PREC_FP32 ? 2 * (size_t)depth * B * n * sizeof(float) : 0;
+    w.dt = 2 * (size_t)B * B * sizeof(float);
+    w.inv = N * (size_t)(3 * (B / 32) * 1024) * sizeof(float);
+    w.inner = 0;
+    if (B > (uint32_t)kMaxLeafLU) {
+        // recursion scratch per level h = B/2, B/4, .., 128: Lt and DT (h x h
+        // each) and the 3 h/32 inverses of 32 x 32
+        size_t need = 0;
+        for (uint32_t h = B / 2; h >= (uint32_t)kMaxLeafLU; h /= 2)
+            need += 2 * (size_t)h * h + 3 * (h / 32) * 1024;
+        w.inner = need * sizeof(float);
+    }
+    w.lp = prec == HPL_PREC_FP32 ? 0 : 2 * (size_t)depth * n * B * lp_elem_bytes(prec);
+    w.sc = prec == HPL_PREC_FP8 ? 2 * (size_t)depth * (n / 32) * sizeof(float) : 0;
+    w.status = 64;
+    return w;
+}
+
+IrPlan plan_ir(uint64_t n, int nrhs) {
+    IrPlan p{};
+    p.work = (size_t)residual_multi_work((int64_t)n, nrhs) * sizeof(double);
+    p.r = (size_t)n * nrhs * sizeof(double);
+    p.xs = (size_t)n * nrhs * sizeof(float);
+    p.out = 4 * (size_t)nrhs * sizeof(double);
+    return p;
+}
+
+hpl_status init_streams(hpl_ctx* c) {
+    if (hipStreamCreateWithFlags(&c->s0, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->s1, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->e0) != hipSuccess || hipEventCreate(&c->e1) != hipSuccess ||
+        hipEventCreateWithFlags(&c->evA, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->evP, hipEventDisableTiming) != hipSuccess)
+        return fail(c, HPL_ERR_HIP, "stream or event creation failed");
+    return HPL_SUCCESS;
+}
+
+hpl_status ir_solve(hpl_ctx* c, int nrhs, const double* dB, double* dX, int max_iter, double tol,
+                    hpl_ir_result* res) {
+    if (nrhs < 1 || nrhs > 16) return fail(c, HPL_ERR_INVALID_ARG, "nrhs %d not in [1,16]", nrhs);
+    const int64_t n = (int64_t)c->n;
+    const IrPlan p = plan_ir(c->n, nrhs);
+    hpl_status st = ensure(c, c->dWork, c->capWork, p.work);
+    if (st) return st;
+    st = ensure(c, c->dIrR, c->capIrR, p.r);
+    if (st) return st;
+    st = ensure(c, c->dIrXs, c->capIrXs, p.xs);
+    if (st) return st;
+    st = ensure(c, c->dIrOut, c->capIrOut, p.out);
+    if (st) return st;
+    const int B = (int)c->B;
+    const int64_t inv_stride = 3 * (B / 32) * 1024;
+    double sc_in[16], sc_out[16], h[64];
+    int iters[16];
+    for (int j = 0; j < nrhs; ++j) {
+        sc_in[j] = sc_out[j] = 1.0;
+        iters[j] = 0;
+    }
+    // x0 = F^-1 b in FP32, promoted
+    HIPCHK(c, launch_d2f_multi(dB, n, c->dIrXs, n, n, nrhs, sc_in, c->s0));
+    HIPCHK(c, launch_gesl_multi(c->dA, (int64_t)c->lda, n, B, c->dInv, inv_stride, c->dIrXs, n,
+                                nrhs, c->s0));
+    HIPCHK(c, launch_axpy_multi(dX, n, c->dIrXs, n, n, nrhs, sc_out, 1, c->s0));
+    for (int it = 0;; ++it) {
+        HIPCHK(c, launch_residual_multi(c->dA0, (int64_t)c->lda, n, dX, n, dB, n, nrhs, c->dWork,
+                                        c->dIrR, n, c->dIrOut, c->s0));
+        HIPCHK(c, hipMemcpyAsync(h, c->dIrOut, 4 * nrhs * sizeof(double), hipMemcpyDeviceToHost,
+                                 c->s0));
+        HIPCHK(c, hipStreamSynchronize(c->s0));
+        bool any = false;
+        for (int j = 0; j < nrhs; ++j) {
+            const double* hj = h + 4 * j;
+            const double berr = hj[0] / (hj[1] * hj[2] + hj[3]);
+            res[j].iterations = iters[j];
+            res[j].converged = berr <= tol ? 1 : 0;
+            res[j].backward_error = berr;
+            res[j].residual_inf = hj[0];
+            res[j].norm_a = hj[1];
+            res[j].norm_x = hj[2];
+            res[j].norm_b = hj[3];
+            res[j].gefa_s = res[j].total_s = 0.0;
+            // a column is corrected while it misses tol, has corrections left
+            // and a finite, non-zero residual
+            const bool active = !(berr <= tol) && it < max_iter && hj[0] > 0.0 &&
+                                hj[0] < HUGE_VAL;
+            sc_in[j] = active ? 1.0 / hj[0] : 0.0;
+            sc_out[j] = active ? hj[0] : 0.0;
+            iters[j] += active ? 1 : 0;
+            any = any || active;
+        }
+        if (!any) break;
+        // correction: F d = r / ||r|| in FP32, x += ||r|| d in FP64 (inactive
+        // columns get scale 0 and stay unchanged)
+        HIPCHK(c, launch_d2f_multi(c->dIrR, n, c->dIrXs, n, n, nrhs, sc_in, c->s0));
+        HIPCHK(c, launch_gesl_multi(c->dA, (int64_t)c->lda, n, B, c->dInv, inv_stride, c->dIrXs,
+                                    n, nrhs, c->s0));
+        HIPCHK(c, launch_axpy_multi(dX, n, c->dIrXs, n, n, nrhs, sc_out, 0, c->s0));
+    }
+    for (int j = 0; j < nrhs; ++j)
+        if (!res[j].converged)
+            return fail(c, HPL_ERR_NOT_CONVERGED,
+                        "refinement of column %d stopped after %d iterations at backward error "
+                        "%.3e (tol %.3e)",
+                        j, res[j].iterations, res[j].backward_error, tol);
+    return HPL_SUCCESS;
+}
+
+}  // namespace hplx
 
 extern "C" {
 
@@ -386,11 +465,7 @@ hpl_status hpl_create(int device, hpl_ctx** out) {
     if (const char* e = getenv("HPL_LOOKAHEAD")) c->lookahead = atoi(e) != 0;
     if (const char* e = getenv("HPL_UPDATE_DEPTH"))
         c->depth_fp32 = c->depth_lp = std::max(1, std::min(8, atoi(e)));
-    if (hipStreamCreateWithFlags(&c->s0, hipStreamNonBlocking) != hipSuccess ||
-        hipStreamCreateWithFlags(&c->s1, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&c->e0) != hipSuccess || hipEventCreate(&c->e1) != hipSuccess ||
-        hipEventCreateWithFlags(&c->evA, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c->evP, hipEventDisableTiming) != hipSuccess) {
+    if (hplx::init_streams(c) != HPL_SUCCESS) {
         hpl_destroy(c);
         return HPL_ERR_HIP;
     }
@@ -403,11 +478,14 @@ void hpl_destroy(hpl_ctx* c) {
     (void)hipSetDevice(c->device);
     if (c->s0) (void)hipStreamSynchronize(c->s0);
     if (c->s1) (void)hipStreamSynchronize(c->s1);
-    for (void* p : {(void*)c->dA, (void*)c->dA0, (void*)c->db, (void*)c->dx, (void*)c->db64,
-                    (void*)c->dxt, (void*)c->dLt, (void*)c->dDT, (void*)c->dInv,
-                    (void*)c->dInner, (void*)c->dStatus, (void*)c->dWork, (void*)c->dOut4,
-                    c->dLpT, c->dLpL, (void*)c->dR64, (void*)c->dX64})
-        if (p) (void)hipFree(p);
+    if (!c->external)
+        for (void* p : {(void*)c->dA, (void*)c->dA0, (void*)c->db, (void*)c->dx, (void*)c->db64,
+                        (void*)c->dxt, (void*)c->dLt, (void*)c->dDT, (void*)c->dInv,
+                        (void*)c->dInner, (void*)c->dStatus, (void*)c->dWork, (void*)c->dOut4,
+                        c->dLpT, c->dLpL, (void*)c->dScT, (void*)c->dScL, (void*)c->dX64,
+                        (void*)c->dIrR, (void*)c->dIrXs, (void*)c->dIrOut, (void*)c->dRhsB,
+                        (void*)c->dRhsX})
+            if (p) (void)hipFree(p);
     for (auto& e : c->ev) {
         (void)hipEventDestroy(e.first);
         (void)hipEventDestroy(e.second);
@@ -496,10 +574,12 @@ hpl_status hpl_gefa_dev(hpl_ctx* c, uint32_t block, hpl_prec prec, hpl_timing* t
         return fail(c, HPL_ERR_INVALID_ARG,
                     "block size %u invalid for n=%llu (power of two in [32,512] dividing n)",
                     block, (unsigned long long)c->n);
-    if (prec != HPL_PREC_FP32 && prec != HPL_PREC_BF16 && prec != HPL_PREC_FP16)
+    if (prec != HPL_PREC_FP32 && prec != HPL_PREC_BF16 && prec != HPL_PREC_FP16 &&
+        prec != HPL_PREC_FP8)
         return fail(c, HPL_ERR_UNSUPPORTED, "precision %d not available", (int)prec);
     if (prec != HPL_PREC_FP32 && block % 64)
-        return fail(c, HPL_ERR_INVALID_ARG, "16-bit trailing updates need a block multiple of 64");
+        return fail(c, HPL_ERR_INVALID_ARG,
+                    "low-precision trailing updates need a block multiple of 64");
     HIPCHK(c, hipSetDevice(c->device));
     c->prec = (int)prec;
     c->depth = prec == HPL_PREC_FP32 ? c->depth_fp32 : c->depth_lp;
@@ -608,7 +688,10 @@ hpl_status hpl_residual(hpl_ctx* c, double out[4]) {
     hpl_status st = ensure(c, c->dWork, c->capWork,
                            2 * (size_t)residual_chunks(n) * n * sizeof(double));
     if (st) return st;
-    if (!c->dOut4) HIPCHK(c, hipMalloc(&c->dOut4, 4 * sizeof(double)));
+    if (!c->dOut4) {
+        if (c->external) return fail(c, HPL_ERR_STATE, "no residual buffer in external mode");
+        HIPCHK(c, hipMalloc(&c->dOut4, 4 * sizeof(double)));
+    }
     HIPCHK(c, launch_residual_f32x(c->dA0, (int64_t)c->lda, n, c->dx, c->db, c->dWork, nullptr,
                                    c->dOut4, c->s0));
     HIPCHK(c, hipMemcpyAsync(out, c->dOut4, 4 * sizeof(double), hipMemcpyDeviceToHost, c->s0));
@@ -623,73 +706,62 @@ hpl_status hpl_solve_ir(hpl_ctx* c, uint32_t block, hpl_prec prec, int max_iter,
     if (max_iter < 0) return fail(c, HPL_ERR_INVALID_ARG, "max_iter < 0");
     HIPCHK(c, hipSetDevice(c->device));
     const int64_t n = (int64_t)c->n;
-    hpl_status st = ensure(c, c->dWork, c->capWork, 2 * (size_t)residual_chunks(n) * n * sizeof(double));
+    hpl_status st = ensure(c, c->dX64, c->capIR, n * sizeof(double));
     if (st) return st;
-    if (!c->dOut4) HIPCHK(c, hipMalloc(&c->dOut4, 4 * sizeof(double)));
-    if ((size_t)n * sizeof(double) > c->capIR) {
-        if (c->dR64) (void)hipFree(c->dR64);
-        if (c->dX64) (void)hipFree(c->dX64);
-        c->dR64 = c->dX64 = nullptr;
-        c->capIR = 0;
-        HIPCHK(c, hipMalloc(&c->dR64, n * sizeof(double)));
-        HIPCHK(c, hipMalloc(&c->dX64, n * sizeof(double)));
-        c->capIR = n * sizeof(double);
-    }
     // fresh matrix from the kept original (not timed)
     HIPCHK(c, hipMemcpyAsync(c->dA, c->dA0, n * c->lda * sizeof(float), hipMemcpyDeviceToDevice,
                              c->s0));
     hipEvent_t t0, t1;
     HIPCHK(c, hipEventCreate(&t0));
-    HIPCHK(c, hipEventCreate(&t1));
-    HIPCHK(c, hipEventRecord(t0, c->s0));
-    hpl_timing tg{};
-    st = hpl_gefa_dev(c, block, prec, &tg);
-    if (st) {
+    if (hipEventCreate(&t1) != hipSuccess) {
         (void)hipEventDestroy(t0);
-        (void)hipEventDestroy(t1);
-        return st;
+        return fail(c, HPL_ERR_HIP, "hipEventCreate failed");
     }
-    const int B = (int)c->B;
-    const int64_t inv_stride = 3 * (B / 32) * 1024;
-    // x = F^-1 b in FP32, then promote
-    HIPCHK(c, launch_d2f_scaled(c->db64, c->dx, n, 1.0, c->s0));
-    HIPCHK(c, launch_gesl(c->dA, (int64_t)c->lda, n, B, c->dInv, inv_stride, c->dx, c->s0));
-    HIPCHK(c, launch_axpy_f2d(c->dX64, c->dx, n, 1.0, 1, c->s0));
-    int it = 0;
-    double berr = 0.0, h[4];
-    for (;; ++it) {
-        HIPCHK(c, launch_residual_f64x(c->dA0, (int64_t)c->lda, n, c->dX64, c->db64, c->dWork,
-                                       c->dR64, c->dOut4, true, c->s0));
-        HIPCHK(c, hipMemcpyAsync(h, c->dOut4, sizeof h, hipMemcpyDeviceToHost, c->s0));
-        HIPCHK(c, hipStreamSynchronize(c->s0));
-        berr = h[0] / (h[1] * h[2] + h[3]);
-        if (berr <= tol || it >= max_iter || !(h[0] > 0.0)) break;
-        // correction: solve F d = r / ||r|| in FP32, x += ||r|| d in FP64
-        HIPCHK(c, launch_d2f_scaled(c->dR64, c->dx, n, 1.0 / h[0], c->s0));
-        HIPCHK(c, launch_gesl(c->dA, (int64_t)c->lda, n, B, c->dInv, inv_stride, c->dx, c->s0));
-        HIPCHK(c, launch_axpy_f2d(c->dX64, c->dx, n, h[0], 0, c->s0));
+    hpl_timing tg{};
+    st = hipEventRecord(t0, c->s0) == hipSuccess ? HPL_SUCCESS
+                                                 : fail(c, HPL_ERR_HIP, "hipEventRecord failed");
+    if (!st) st = hpl_gefa_dev(c, block, prec, &tg);
+    if (!st) {
+        st = hplx::ir_solve(c, 1, c->db64, c->dX64, max_iter, tol, out);
+        if (st == HPL_SUCCESS || st == HPL_ERR_NOT_CONVERGED) {
+            float ms = 0.f;
+            if (hipEventRecord(t1, c->s0) != hipSuccess || hipEventSynchronize(t1) != hipSuccess ||
+                hipEventElapsedTime(&ms, t0, t1) != hipSuccess)
+                st = fail(c, HPL_ERR_HIP, "refinement timing failed");
+            out->gefa_s = tg.gefa_s;
+            out->total_s = ms * 1e-3;
+        }
     }
-    HIPCHK(c, hipEventRecord(t1, c->s0));
-    HIPCHK(c, hipEventSynchronize(t1));
-    float ms = 0.f;
-    HIPCHK(c, hipEventElapsedTime(&ms, t0, t1));
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
-    out->iterations = it;
-    out->converged = berr <= tol ? 1 : 0;
-    out->backward_error = berr;
-    out->residual_inf = h[0];
-    out->norm_a = h[1];
-    out->norm_x = h[2];
-    out->norm_b = h[3];
-    out->gefa_s = tg.gefa_s;
-    out->total_s = ms * 1e-3;
-    c->factored = true;
-    if (!out->converged)
-        return fail(c, HPL_ERR_NOT_CONVERGED,
-                    "refinement stopped after %d iterations at backward error %.3e (tol %.3e)", it,
-                    berr, tol);
-    return HPL_SUCCESS;
+    return st;
+}
+
+/* Solve nrhs further right-hand sides with the resident factor and refine
+ * each in FP64 against the kept original. */
+hpl_status hpl_solve_rhs64(hpl_ctx* c, int nrhs, const double* b, uint64_t ldb, double* x,
+                           uint64_t ldx, int max_iter, double tol, hpl_ir_result* res) {
+    if (!c) return HPL_ERR_INVALID_ARG;
+    if (!b || !x || !res || nrhs < 1 || nrhs > 16 || max_iter < 0)
+        return fail(c, HPL_ERR_INVALID_ARG, "bad hpl_solve_rhs64 arguments (nrhs=%d)", nrhs);
+    if (!c->factored || !c->have_orig)
+        return fail(c, HPL_ERR_STATE, "hpl_solve_rhs64 needs a factor of a kept original");
+    const uint64_t n = c->n;
+    if (ldb < n || ldx < n) return fail(c, HPL_ERR_INVALID_ARG, "ldb or ldx < n");
+    HIPCHK(c, hipSetDevice(c->device));
+    const size_t bytes = n * (size_t)nrhs * sizeof(double);
+    hpl_status st = ensure(c, c->dRhsB, c->capRhsB, bytes);
+    if (st) return st;
+    st = ensure(c, c->dRhsX, c->capRhsX, bytes);
+    if (st) return st;
+    HIPCHK(c, hipMemcpy2DAsync(c->dRhsB, n * sizeof(double), b, ldb * sizeof(double),
+                               n * sizeof(double), (size_t)nrhs, hipMemcpyHostToDevice, c->s0));
+    st = hplx::ir_solve(c, nrhs, c->dRhsB, c->dRhsX, max_iter, tol, res);
+    if (st && st != HPL_ERR_NOT_CONVERGED) return st;
+    HIPCHK(c, hipMemcpy2DAsync(x, ldx * sizeof(double), c->dRhsX, n * sizeof(double),
+                               n * sizeof(double), (size_t)nrhs, hipMemcpyDeviceToHost, c->s0));
+    HIPCHK(c, hipStreamSynchronize(c->s0));
+    return st;
 }
 
 hpl_status hpl_download_solution64(hpl_ctx* c, double* x) {

@@ -1,0 +1,121 @@
+// The solver context behind the C ABI (internal to libhpl_hip.so).
+//
+// A context owns two HIP streams and the device buffers of one resident
+// system: the matrix being factorised (dA), an untouched copy for residuals
+// (dA0), vectors, the factorisation workspaces and the per-block inverses.
+// In its own mode every buffer is allocated on demand (hipMalloc) and freed by
+// hpl_destroy.  In external mode (the serving engine) the buffers are regions
+// of a larger allocation set up by the owner: nothing is allocated or freed,
+// and a workspace that would have to grow is an error.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/hpl_hip.h"
+
+struct hpl_ctx {
+    int device = 0;
+    bool external = false;
+    hipStream_t s0 = nullptr;   // trailing updates (and everything else)
+    hipStream_t s1 = nullptr;   // look-ahead panel factorisation
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipEvent_t evA = nullptr, evP = nullptr;  // next-panel region updated / panel ready
+    bool lookahead = true;
+    // Bulk trailing update every `depth` steps (K = depth * B).  The FP32
+    // update is MFMA-bound, so 1; the 16-bit update is bound by the FP32 C
+    // traffic (8 bytes per element per pass), which `depth` divides.
+    int depth = 1;
+    int depth_fp32 = 1, depth_lp = 8;
+    std::string err;
+
+    // resident system
+    uint64_t n = 0, lda = 0;
+    float* dA = nullptr;
+    size_t capA = 0;
+    float* dA0 = nullptr;  // untouched copy (keep_original)
+    size_t capA0 = 0;
+    bool keep_orig = false;
+    bool have_orig = false;
+    float* db = nullptr;
+    float* dx = nullptr;
+    double* db64 = nullptr;
+    double* dxt = nullptr;
+    size_t capVec = 0;
+
+    // GEFA workspaces
+    uint32_t B = 0;             // block size of the resident factor (0: none)
+    float* dLt = nullptr;       // transposed left strips, 2 * depth slots of B x n
+    size_t capLt = 0;
+    float* dDT = nullptr;       // transposed diagonal block, 2 x (B x B), by step parity
+    size_t capDT = 0;
+    float* dInv = nullptr;      // per block: 3 * (B/32) inverses of 32 x 32
+    size_t capInv = 0;
+    float* dInner = nullptr;    // recursion scratch for blocks > 128
+    size_t capInner = 0;
+    void* dLpT = nullptr;       // low-precision k-contiguous top-strip copies, 2*depth slots of n x B
+    void* dLpL = nullptr;       // low-precision k-contiguous left-strip copies, same layout
+    size_t capLpT = 0, capLpL = 0;
+    float* dScT = nullptr;      // FP8 block scales of the copies, 2*depth slots of n/32
+    float* dScL = nullptr;
+    size_t capScT = 0, capScL = 0;
+    int prec = 0;               // trailing-update precision of the current factorisation
+    int* dStatus = nullptr;
+    size_t capStatus = 0;
+    double* dWork = nullptr;    // residual partial sums
+    size_t capWork = 0;
+    double* dX64 = nullptr;     // refined solution of hpl_solve_ir (FP64)
+    size_t capIR = 0;
+    double* dOut4 = nullptr;
+    // refinement of nrhs columns (leading dimension n): residuals (FP64),
+    // FP32 corrections, norms (4 per column)
+    double* dIrR = nullptr;
+    float* dIrXs = nullptr;
+    double* dIrOut = nullptr;
+    size_t capIrR = 0, capIrXs = 0, capIrOut = 0;
+    // right-hand sides and solutions of hpl_solve_rhs64
+    double* dRhsB = nullptr;
+    double* dRhsX = nullptr;
+    size_t capRhsB = 0, capRhsX = 0;
+    bool factored = false;
+
+    // roofline accounting of the trailing update
+    bool prof = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+    size_t ev_used = 0;
+    double prof_flops = 0.0;
+};
+
+namespace hplx {
+
+hpl_status fail(hpl_ctx* c, hpl_status st, const char* fmt, ...);
+
+// Streams and events of a context (default priority, non-blocking).
+hpl_status init_streams(hpl_ctx* c);
+
+// Device bytes of the factorisation workspaces of an n x n system at block B
+// with the given trailing-update precision and update depth.
+struct WsPlan {
+    size_t lt, dt, inner, lp, sc, status;
+    size_t inv;  // per-block inverses (kept with the factor)
+};
+WsPlan plan_factor_ws(uint64_t n, uint32_t B, int prec, int depth);
+// Device bytes of the refinement buffers for nrhs columns of order n.
+struct IrPlan {
+    size_t work, r, xs, out;
+};
+IrPlan plan_ir(uint64_t n, int nrhs);
+
+// Classic refinement of nrhs columns with the resident factor: dB (FP64,
+// column j at dB + j * n) -> dX.  res[j] receives column j's result (its
+// gefa_s / total_s are left 0).  Returns HPL_ERR_NOT_CONVERGED when a column
+// misses tol.
+hpl_status ir_solve(hpl_ctx* c, int nrhs, const double* dB, double* dX, int max_iter, double tol,
+                    hpl_ir_result* res);
+
+uint64_t device_lda(uint64_t n);
+
+}  // namespace hplx

@@ -1,4 +1,5 @@
-// Mixed-precision trailing update: BF16 or FP16 operands, FP32 accumulation.
+// Mixed-precision trailing update: BF16, FP16 or scaled FP8 operands, FP32
+// accumulation.
 //
 // C[r][c] += sum_k L[r][k] * T[c][k]     (FP32 C, 16-bit L and T)
 // L and T are the low-precision copies of the left and top strips written by
@@ -165,7 +166,151 @@ hipError_t launch_t(float* C, int64_t ldc, const void* L, int64_t l_ld, int64_t 
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// FP8 E4M3 operands with per-block power-of-two scales.
+//
+// v_mfma_f32_32x32x16_fp8_fp8: lane l supplies row (l % 32) of the A block and
+// column (l % 32) of the B block, 8 consecutive k (8 * (l / 32) .. + 7) as 8
+// bytes.  A step (panel) of Bblk k values carries one scale per 32 rows of L
+// and per 32 columns of T, so each 32 x 32 accumulator tile of a step has the
+// single factor scL * scT: the step is accumulated unscaled in `part` and
+// added to the FP32 result with that factor when the step ends.
+constexpr int BK8 = 64;          // bytes (= k values) per slab
+constexpr int LDR8 = BK8 + 16;   // padded LDS row, bytes
+
+__global__ __launch_bounds__(NT, 2) void update_fp8_kernel(
+    float* __restrict__ C, int64_t ldc, const uint8_t* __restrict__ L, int64_t l_ld,
+    int64_t l_slot, const uint8_t* __restrict__ Tt, int64_t t_ld, int64_t t_slot,
+    const float* __restrict__ scL, const float* __restrict__ scT, int64_t sc_slot, int M, int N,
+    int K, int Bblk) {
+    __shared__ __attribute__((aligned(16))) uint8_t As[2][BM * LDR8];
+    __shared__ __attribute__((aligned(16))) uint8_t Bs[2][BN * LDR8];
+
+    const int tiles_m = (M + BM - 1) / BM, tiles_n = (N + BN - 1) / BN;
+    int tm, tn;
+    tile_coords(blockIdx.x, gridDim.x, tiles_m, tiles_n, tm, tn);
+    const int m0 = tm * BM, n0 = tn * BN;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int lr = lane & 31, lh = lane >> 5;
+    const bool full = (m0 + BM <= M) && (n0 + BN <= N);
+
+    // global -> LDS: 2 chunks of 16 bytes per operand per thread
+    uint4 ra[2], rb[2];
+    auto load_slab = [&](int k0) {
+        const int64_t ks = k0 / Bblk, ko = k0 % Bblk;
+        const uint8_t* lb = L + ks * l_slot + ko;
+        const uint8_t* tb = Tt + ks * t_slot + ko;
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            const int f = tid + NT * s, row = f >> 2, ch = (f & 3) * 16;
+            ra[s] = (full || m0 + row < M)
+                        ? *reinterpret_cast<const uint4*>(lb + (int64_t)(m0 + row) * l_ld + ch)
+                        : make_uint4(0, 0, 0, 0);
+            rb[s] = (full || n0 + row < N)
+                        ? *reinterpret_cast<const uint4*>(tb + (int64_t)(n0 + row) * t_ld + ch)
+                        : make_uint4(0, 0, 0, 0);
+        }
+    };
+    auto store_slab = [&](int buf) {
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            const int f = tid + NT * s, row = f >> 2, ch = (f & 3) * 16;
+            *reinterpret_cast<uint4*>(&As[buf][row * LDR8 + ch]) = ra[s];
+            *reinterpret_cast<uint4*>(&Bs[buf][row * LDR8 + ch]) = rb[s];
+        }
+    };
+
+    f32x16 acc[2][2], part[2][2];
+    const int rbase = m0 + wm * WM, cbase = n0 + wn * WN;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int row = rbase + i * 32 + acc_row(r, lane);
+                const int col = cbase + j * 32 + lr;
+                acc[i][j][r] = (full || (row < M && col < N)) ? C[(int64_t)row * ldc + col] : 0.0f;
+                part[i][j][r] = 0.0f;
+            }
+
+    load_slab(0);
+    store_slab(0);
+    __syncthreads();
+    const int nk = K / BK8, per_step = Bblk / BK8;
+    for (int kc = 0; kc < nk; ++kc) {
+        const int buf = kc & 1;
+        const bool more = kc + 1 < nk;
+        load_slab((more ? kc + 1 : kc) * BK8);
+#pragma unroll
+        for (int ks = 0; ks < BK8 / 16; ++ks) {
+            long a[2], b[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+                a[i] = *reinterpret_cast<const long*>(
+                    &As[buf][(wm * WM + i * 32 + lr) * LDR8 + ks * 16 + 8 * lh]);
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+                b[j] = *reinterpret_cast<const long*>(
+                    &Bs[buf][(wn * WN + j * 32 + lr) * LDR8 + ks * 16 + 8 * lh]);
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    part[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8(a[i], b[j], part[i][j],
+                                                                           0, 0, 0);
+        }
+        if ((kc + 1) % per_step == 0) {
+            // end of a step: scale its tiles into the result
+            const int64_t p = kc / per_step;
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                const int rg = (rbase + i * 32) / 32;
+                const float sl = rbase + i * 32 < M ? scL[p * sc_slot + rg] : 0.0f;
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    const int cg = (cbase + j * 32) / 32;
+                    const float f = cbase + j * 32 < N ? sl * scT[p * sc_slot + cg] : 0.0f;
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) {
+                        acc[i][j][r] = fmaf(f, part[i][j][r], acc[i][j][r]);
+                        part[i][j][r] = 0.0f;
+                    }
+                }
+            }
+        }
+        if (more) store_slab(buf ^ 1);
+        __syncthreads();
+    }
+
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int row = rbase + i * 32 + acc_row(r, lane);
+                const int col = cbase + j * 32 + lr;
+                if (full || (row < M && col < N)) C[(int64_t)row * ldc + col] = acc[i][j][r];
+            }
+}
+
 }  // namespace
+
+hipError_t launch_gemm_update_fp8(float* C, int64_t ldc, const void* L, int64_t l_ld,
+                                  int64_t l_slot, const void* Tt, int64_t t_ld, int64_t t_slot,
+                                  const float* scL, const float* scT, int64_t sc_slot, int M,
+                                  int N, int K, int Bblk, hipStream_t s) {
+    if (M <= 0 || N <= 0) return hipSuccess;
+    if (K % Bblk || Bblk % BK8 || (l_ld % 16) || (t_ld % 16) || (M % 32) || (N % 32))
+        return hipErrorInvalidValue;
+    const int tiles = ((M + BM - 1) / BM) * ((N + BN - 1) / BN);
+    update_fp8_kernel<<<tiles, NT, 0, s>>>(C, ldc, static_cast<const uint8_t*>(L), l_ld, l_slot,
+                                           static_cast<const uint8_t*>(Tt), t_ld, t_slot, scL, scT,
+                                           sc_slot, M, N, K, Bblk);
+    return hipGetLastError();
+}
 
 hipError_t launch_gemm_update_lp(int prec, float* C, int64_t ldc, const void* L, int64_t l_ld,
                                  int64_t l_slot, const void* Tt, int64_t t_ld, int64_t t_slot,

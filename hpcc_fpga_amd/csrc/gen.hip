@@ -198,3 +198,178 @@ hipError_t launch_residual_f64x(const float* A, int64_t lda, int64_t n, const do
                                 bool with_norm, hipStream_t s) {
     return launch_residual_t<double, double>(A, lda, n, x, b, work, r, out4, with_norm, s);
 }
+
+// ---------------------------------------------------------------------------
+// Several right-hand sides: R[r] = B[r] - M X[r], column r at offset r * ld.
+// The matrix is read once for all of them; partial sums per chunk of rows
+// (partial[(c * nr + r) * n + i]) are added in a fixed order as above.
+namespace {
+constexpr int kMaxMultiRhs = 16;
+
+__global__ void resid_partial_multi_kernel(const float* __restrict__ A, int64_t lda, int64_t n,
+                                           const double* __restrict__ X, int64_t ldx, int nr,
+                                           double* __restrict__ partial,
+                                           double* __restrict__ pabs) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t c = blockIdx.y;
+    if (i >= n) return;
+    const int64_t j0 = c * RCH, j1 = min(n, j0 + RCH);
+    double s[kMaxMultiRhs];
+#pragma unroll
+    for (int r = 0; r < kMaxMultiRhs; ++r) s[r] = 0.0;
+    double sa = 0.0;
+    for (int64_t j = j0; j < j1; ++j) {
+        const double a = (double)A[j * lda + i];
+#pragma unroll
+        for (int r = 0; r < kMaxMultiRhs; ++r)
+            if (r < nr) s[r] = fma(a, X[r * ldx + j], s[r]);
+        sa += fabs(a);
+    }
+#pragma unroll
+    for (int r = 0; r < kMaxMultiRhs; ++r)
+        if (r < nr) partial[(c * nr + r) * n + i] = s[r];
+    if (pabs) pabs[c * n + i] = sa;
+}
+
+__global__ void zero_kernel(double* p, int count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) p[i] = 0.0;
+}
+
+// grid.y = right-hand side; out[4 r + 0..3] = ||r||, ||M||, ||x||, ||b|| (inf)
+__global__ void resid_final_multi_kernel(int64_t n, int64_t nch, int nr,
+                                         const double* __restrict__ partial,
+                                         const double* __restrict__ pabs,
+                                         const double* __restrict__ X, int64_t ldx,
+                                         const double* __restrict__ Bv, int64_t ldb,
+                                         double* __restrict__ R, int64_t ldr,
+                                         double* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    double rv = 0, av = 0, xv = 0, bv = 0;
+    if (i < n) {
+        double s = 0.0, sa = 0.0;
+        for (int64_t c = 0; c < nch; ++c) {
+            s += partial[(c * nr + r) * n + i];
+            if (pabs) sa += pabs[c * n + i];
+        }
+        const double bi = Bv[r * ldb + i];
+        const double ri = bi - s;
+        if (R) R[r * ldr + i] = ri;
+        rv = fabs(ri); av = sa; xv = fabs(X[r * ldx + i]); bv = fabs(bi);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        rv = fmax(rv, __shfl_xor(rv, o));
+        av = fmax(av, __shfl_xor(av, o));
+        xv = fmax(xv, __shfl_xor(xv, o));
+        bv = fmax(bv, __shfl_xor(bv, o));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomic_max_nonneg(out + 4 * r + 0, rv);
+        atomic_max_nonneg(out + 4 * r + 1, av);
+        atomic_max_nonneg(out + 4 * r + 2, xv);
+        atomic_max_nonneg(out + 4 * r + 3, bv);
+    }
+}
+
+struct ColScales {
+    double v[kMaxMultiRhs];
+};
+
+// Xs[r][i] = (float)(R[r][i] * scale[r])
+__global__ void d2f_multi_kernel(const double* __restrict__ R, int64_t ldr, float* __restrict__ Xs,
+                                 int64_t ldxs, int64_t n, ColScales sc) {
+    const int r = blockIdx.y;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        Xs[r * ldxs + i] = (float)(R[r * ldr + i] * sc.v[r]);
+}
+
+// X[r][i] = scale[r] * D[r][i] (assign) or X[r][i] += scale[r] * D[r][i]
+__global__ void axpy_multi_kernel(double* __restrict__ X, int64_t ldx, const float* __restrict__ D,
+                                  int64_t ldd, int64_t n, ColScales sc, int assign) {
+    const int r = blockIdx.y;
+    const double a = sc.v[r];
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const double d = (double)D[r * ldd + i];
+        X[r * ldx + i] = assign ? a * d : fma(a, d, X[r * ldx + i]);
+    }
+}
+}  // namespace
+
+int64_t residual_multi_work(int64_t n, int nrhs) {
+    return residual_chunks(n) * n * (nrhs + 1);  // doubles
+}
+
+hipError_t launch_residual_multi(const float* A, int64_t lda, int64_t n, const double* X,
+                                 int64_t ldx, const double* Bv, int64_t ldb, int nrhs,
+                                 double* work, double* R, int64_t ldr, double* out,
+                                 hipStream_t s) {
+    if (nrhs < 1 || nrhs > kMaxMultiRhs) return hipErrorInvalidValue;
+    const int64_t nch = residual_chunks(n);
+    double* partial = work;
+    double* pabs = work + nch * n * nrhs;
+    dim3 g1((unsigned)((n + 255) / 256), (unsigned)nch);
+    resid_partial_multi_kernel<<<g1, 256, 0, s>>>(A, lda, n, X, ldx, nrhs, partial, pabs);
+    zero_kernel<<<1, 64, 0, s>>>(out, 4 * nrhs);  // a kernel, not a memset node
+    dim3 g2((unsigned)((n + 255) / 256), (unsigned)nrhs);
+    resid_final_multi_kernel<<<g2, 256, 0, s>>>(n, nch, nrhs, partial, pabs, X, ldx, Bv, ldb, R,
+                                                ldr, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_d2f_multi(const double* R, int64_t ldr, float* Xs, int64_t ldxs, int64_t n,
+                            int nrhs, const double* scale, hipStream_t s) {
+    if (nrhs < 1 || nrhs > kMaxMultiRhs) return hipErrorInvalidValue;
+    ColScales sc{};
+    for (int r = 0; r < nrhs; ++r) sc.v[r] = scale[r];
+    dim3 g((unsigned)std::min<int64_t>((n + 255) / 256, 1024), (unsigned)nrhs);
+    d2f_multi_kernel<<<g, 256, 0, s>>>(R, ldr, Xs, ldxs, n, sc);
+    return hipGetLastError();
+}
+
+hipError_t launch_axpy_multi(double* X, int64_t ldx, const float* D, int64_t ldd, int64_t n,
+                             int nrhs, const double* scale, int assign, hipStream_t s) {
+    if (nrhs < 1 || nrhs > kMaxMultiRhs) return hipErrorInvalidValue;
+    ColScales sc{};
+    for (int r = 0; r < nrhs; ++r) sc.v[r] = scale[r];
+    dim3 g((unsigned)std::min<int64_t>((n + 255) / 256, 1024), (unsigned)nrhs);
+    axpy_multi_kernel<<<g, 256, 0, s>>>(X, ldx, D, ldd, n, sc, assign);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Identity extension of an n x n system to npad x npad: the added rows and
+// columns are zero except for ones on the diagonal, so the factor of the
+// leading block is unchanged and the added unknowns solve to zero.
+namespace {
+__global__ void pad_identity_kernel(float* __restrict__ A, int64_t lda, int64_t n, int64_t npad) {
+    for (int64_t r = blockIdx.y; r < npad; r += gridDim.y) {
+        const int64_t c0 = r < n ? n : 0;  // rows < n: only the added columns
+        for (int64_t c = c0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < npad;
+             c += (int64_t)gridDim.x * blockDim.x)
+            A[r * lda + c] = r == c ? 1.0f : 0.0f;
+    }
+}
+__global__ void fill_f64_kernel(double* __restrict__ p, int64_t count, double v) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+}  // namespace
+
+hipError_t launch_pad_identity(float* A, int64_t lda, int64_t n, int64_t npad, hipStream_t s) {
+    if (npad <= n) return hipSuccess;
+    dim3 g((unsigned)std::min<int64_t>((npad + 255) / 256, 32),
+           (unsigned)std::min<int64_t>(npad, 4096));
+    pad_identity_kernel<<<g, 256, 0, s>>>(A, lda, n, npad);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_f64(double* p, int64_t count, double v, hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    fill_f64_kernel<<<(unsigned)std::min<int64_t>((count + 255) / 256, 1024), 256, 0, s>>>(p, count,
+                                                                                          v);
+    return hipGetLastError();
+}

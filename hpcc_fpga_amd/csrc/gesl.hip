@@ -1,12 +1,14 @@
 // Forward and back substitution with the dgefa-encoded factor
 // ("gesl", LINPACK/src/host/linpack_data.cpp:199-229; the reference runs it on
-// the host, LINPACK/src/host/linpack_benchmark.hpp:78-201).
+// the host, LINPACK/src/host/linpack_benchmark.hpp:78-201), for one or several
+// right-hand sides at once.
 //
 // With D_q the q-th diagonal block and A the row-major factor:
 //   forward  (L y = b, L unit lower, -l stored):  x_e += sum_{p in q} A[p][e] y_p
 //   backward (U x = y):                           x_e -= sum_{p in q} A[p][e] x_p
 // Both read rows of the factor, so every update is a coalesced GEMV (float4
-// loads, the p range split over thread groups and reduced in LDS).
+// loads); with several right-hand sides each factor element read feeds all of
+// them, so the factor is streamed once per sweep whatever the count.
 //
 // One launch per block step.  Workgroup 0 first applies the just-solved block
 // to the NEXT diagonal block's entries and then solves that diagonal system;
@@ -14,55 +16,105 @@
 // parallel.  The diagonal solve works on 32-row sub-blocks: the contribution
 // of solved rows is a dot product spread over the workgroup, and the 32 x 32
 // triangular system is applied as its inverse (computed by the panel prep).
+//
+// Right-hand sides: column r of X at X + r * ldx.  NR is the compile-time
+// capacity (1, 4, 16), nr <= NR the count in use.
 #include "common.hpp"
 #include "kernels.hpp"
+
+#include <algorithm>
 
 namespace {
 
 constexpr int NT = 256;
-constexpr int CHUNK = 256;  // entries per non-diagonal workgroup (64 float4 columns)
 
-// red[0][e] (e < 4*ncols) = sum_{p < B} A[(pbase + p) * lda + ebase + e] * v[p] for the
-// entries ebase .. ebase + 4*ncols - 1 below elim (ncols <= 64).
+template <int NR>
+struct gesl_cfg {
+    static constexpr int MAXB = NR >= 16 ? 256 : 512;       // largest block
+    static constexpr int CH = NR >= 8 ? 1024 : NR >= 4 ? 512 : 256;  // entries per off-diagonal WG
+};
+
+// sum_{p < B} A[(pbase + p) * lda + e] * v[r][p] for the entries
+// e = ebase .. ebase + 4*ncols - 1 below elim, for r < nr, added (SIGN=+1) or
+// subtracted (SIGN=-1) to X[r][e].  The p range is split over NT / ncols
+// thread groups and reduced through red (1024 floats).  Every thread of the
+// workgroup must call it.
+template <int NR, int SIGN>
 __device__ __forceinline__ void block_gemv(const float* __restrict__ A, int64_t lda, int64_t pbase,
                                            int64_t ebase, int ncols, int B, const float* v,
-                                           float (*red)[4 * 64], int64_t elim) {
+                                           int vstride, int nr, float* red, int64_t elim,
+                                           float* __restrict__ X, int64_t ldx) {
     const int t = threadIdx.x;
-    const int groups = min(NT / ncols, NT / 16);  // p-groups (rows of red)
+    const int groups = NT / ncols;
     const int col = t % ncols, g = t / ncols;
-    f32x4 a = {0, 0, 0, 0};
+    f32x4 a[NR];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) a[r] = f32x4{0, 0, 0, 0};
     const int64_t e = ebase + 4 * col;
-    if (g < groups && e < elim) {
+    const bool live = e < elim;  // elim is a multiple of 4
+    if (live) {
         const float* src = A + pbase * lda + e;
-#pragma unroll 8
+#pragma unroll 4
         for (int p = g; p < B; p += groups) {
             const f32x4 m = *reinterpret_cast<const f32x4*>(src + (int64_t)p * lda);
-            const float vp = v[p];
-            a[0] = fmaf(m[0], vp, a[0]);
-            a[1] = fmaf(m[1], vp, a[1]);
-            a[2] = fmaf(m[2], vp, a[2]);
-            a[3] = fmaf(m[3], vp, a[3]);
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                if (r < nr) {
+                    const float vp = v[r * vstride + p];
+                    a[r][0] = fmaf(m[0], vp, a[r][0]);
+                    a[r][1] = fmaf(m[1], vp, a[r][1]);
+                    a[r][2] = fmaf(m[2], vp, a[r][2]);
+                    a[r][3] = fmaf(m[3], vp, a[r][3]);
+                }
+            }
         }
     }
-    if (g < groups) *reinterpret_cast<f32x4*>(&red[g][4 * col]) = a;
-    __syncthreads();
-    if (t < 4 * ncols) {
-        float s = 0.0f;
-        for (int gg = 0; gg < groups; ++gg) s += red[gg][t];
-        red[0][t] = s;
+    if (groups == 1) {
+        if (live) {
+#pragma unroll
+            for (int r = 0; r < NR; ++r) {
+                if (r < nr) {
+                    f32x4* dst = reinterpret_cast<f32x4*>(X + r * ldx + e);
+                    f32x4 x = *dst;
+                    x[0] += SIGN * a[r][0];
+                    x[1] += SIGN * a[r][1];
+                    x[2] += SIGN * a[r][2];
+                    x[3] += SIGN * a[r][3];
+                    *dst = x;
+                }
+            }
+        }
+        return;
     }
-    __syncthreads();
+    for (int r = 0; r < nr; ++r) {
+        f32x4 ar = a[0];
+#pragma unroll
+        for (int rr = 1; rr < NR; ++rr)
+            if (rr == r) ar = a[rr];
+        *reinterpret_cast<f32x4*>(&red[g * 4 * ncols + 4 * col]) = ar;
+        __syncthreads();
+        if (t < 4 * ncols && ebase + t < elim) {
+            float s = 0.0f;
+            for (int gg = 0; gg < groups; ++gg) s += red[gg * 4 * ncols + t];
+            X[r * ldx + ebase + t] += SIGN * s;
+        }
+        __syncthreads();
+    }
 }
 
-// Solve the B x B diagonal system starting at Qoff in x (in place), forward
-// (unit lower) or backward (upper).  256 threads.
-template <bool FWD>
+// Solve the B x B diagonal system starting at Qoff for nr right-hand sides,
+// forward (unit lower) or backward (upper), in place in X.
+template <int NR, bool FWD>
 __device__ void diag_solve(const float* __restrict__ A, int64_t lda, int64_t Qoff, int B,
-                           const float* __restrict__ invT, float* __restrict__ x, float* ybuf,
-                           float (*part)[32]) {
+                           const float* __restrict__ invT, float* __restrict__ X, int64_t ldx,
+                           int nr, float* ybuf, float* part, float* zbuf) {
+    constexpr int MAXB = gesl_cfg<NR>::MAXB;
     const int t = threadIdx.x;
     const int NQ = B / 32;
-    for (int e = t; e < B; e += NT) ybuf[e] = x[Qoff + e];
+    for (int idx = t; idx < nr * B; idx += NT) {
+        const int r = idx / B, e = idx % B;
+        ybuf[r * MAXB + e] = X[r * ldx + Qoff + e];
+    }
     __syncthreads();
     for (int step = 0; step < NQ; ++step) {
         const int s = FWD ? step : NQ - 1 - step;
@@ -70,122 +122,175 @@ __device__ void diag_solve(const float* __restrict__ A, int64_t lda, int64_t Qof
         // rows already solved: forward p < 32 s ; backward p >= 32 (s+1)
         const int p_lo = FWD ? 0 : 32 * (s + 1);
         const int p_hi = FWD ? 32 * s : B;
-        float a = 0.0f;
+        float a[NR];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) a[r] = 0.0f;
         const float* col = A + Qoff * lda + Qoff + 32 * s + i;
 #pragma unroll 4
-        for (int p = p_lo + g; p < p_hi; p += 8) a = fmaf(col[(int64_t)p * lda], ybuf[p], a);
-        part[g][i] = a;
+        for (int p = p_lo + g; p < p_hi; p += 8) {
+            const float m = col[(int64_t)p * lda];
+#pragma unroll
+            for (int r = 0; r < NR; ++r)
+                if (r < nr) a[r] = fmaf(m, ybuf[r * MAXB + p], a[r]);
+        }
+#pragma unroll
+        for (int r = 0; r < NR; ++r)
+            if (r < nr) part[(g * NR + r) * 32 + i] = a[r];
         __syncthreads();
-        if (t < 32) {
+        for (int idx = t; idx < 32 * nr; idx += NT) {
+            const int r = idx >> 5, ii = idx & 31;
             float z = 0.0f;
 #pragma unroll
-            for (int gg = 0; gg < 8; ++gg) z += part[gg][t];
-            z = FWD ? ybuf[32 * s + t] + z : ybuf[32 * s + t] - z;
-            part[0][t] = z;
+            for (int gg = 0; gg < 8; ++gg) z += part[(gg * NR + r) * 32 + ii];
+            const float yv = ybuf[r * MAXB + 32 * s + ii];
+            zbuf[r * 32 + ii] = FWD ? yv + z : yv - z;
         }
         __syncthreads();
-        if (t < 32) {
-            // y_s = X z with X^T stored: X[t][j] = invT[j][t]
+        for (int idx = t; idx < 32 * nr; idx += NT) {
+            const int r = idx >> 5, ii = idx & 31;
+            // y_s = X z with X^T stored: X[ii][j] = invT[j][ii]
             float y = 0.0f;
-            const float* X = invT + s * 1024;
+            const float* Xs = invT + s * 1024;
 #pragma unroll 8
-            for (int j = 0; j < 32; ++j) y = fmaf(X[j * 32 + t], part[0][j], y);
-            ybuf[32 * s + t] = y;
+            for (int j = 0; j < 32; ++j) y = fmaf(Xs[j * 32 + ii], zbuf[r * 32 + j], y);
+            ybuf[r * MAXB + 32 * s + ii] = y;
         }
         __syncthreads();
     }
-    for (int e = t; e < B; e += NT) x[Qoff + e] = ybuf[e];
+    for (int idx = t; idx < nr * B; idx += NT) {
+        const int r = idx / B, e = idx % B;
+        X[r * ldx + Qoff + e] = ybuf[r * MAXB + e];
+    }
 }
 
+template <int NR>
+struct gesl_smem {
+    float ybuf[NR * gesl_cfg<NR>::MAXB];  // diagonal system being solved
+    float vs[NR * gesl_cfg<NR>::MAXB];    // the solved block applied by this step
+    float part[8 * NR * 32];
+    float zbuf[NR * 32];
+    float red[1024];
+};
+
 // Forward step q (q = -1 .. N-2): apply solved block q, solve block q+1.
+template <int NR>
 __global__ __launch_bounds__(NT) void gesl_fwd_kernel(const float* __restrict__ A, int64_t lda,
                                                       int64_t n, int B, int q,
                                                       const float* __restrict__ inv,
-                                                      int64_t inv_stride, float* __restrict__ x) {
-    __shared__ float ybuf[512];
-    __shared__ float part[8][32];
-    __shared__ float ys[512];
-    __shared__ float red[NT / 16][4 * 64];
+                                                      int64_t inv_stride, float* __restrict__ X,
+                                                      int64_t ldx, int nr) {
+    constexpr int MAXB = gesl_cfg<NR>::MAXB, CH = gesl_cfg<NR>::CH;
+    __shared__ gesl_smem<NR> sm;
     const int t = threadIdx.x;
     const int NQ = B / 32;
     const int64_t qoff = (int64_t)q * B;
     if (q >= 0) {
-        for (int e = t; e < B; e += NT) ys[e] = x[qoff + e];
+        for (int idx = t; idx < nr * B; idx += NT) {
+            const int r = idx / B, e = idx % B;
+            sm.vs[r * MAXB + e] = X[r * ldx + qoff + e];
+        }
         __syncthreads();
     }
     if (blockIdx.x == 0) {
         const int64_t Q = (int64_t)(q + 1) * B;
         if (q >= 0) {
             // entries of block q+1: x_e += sum_p A[qoff+p][e] y_p
-            for (int e0 = 0; e0 < B; e0 += 256) {
-                const int ncols = min(64, (B - e0) / 4);
-                block_gemv(A, lda, qoff, Q + e0, ncols, B, ys, red, n);
-                if (t < 4 * ncols) x[Q + e0 + t] += red[0][t];
-                __syncthreads();
+            for (int e0 = 0; e0 < B; e0 += 4 * NT / 2) {
+                const int ncols = min(NT / 2, (B - e0) / 4);
+                block_gemv<NR, 1>(A, lda, qoff, Q + e0, ncols, B, sm.vs, MAXB, nr, sm.red, n, X,
+                                  ldx);
             }
+            __syncthreads();
         }
-        diag_solve<true>(A, lda, Q, B, inv + (int64_t)(q + 1) * inv_stride + NQ * 1024, x, ybuf,
-                         part);
+        diag_solve<NR, true>(A, lda, Q, B, inv + (int64_t)(q + 1) * inv_stride + NQ * 1024, X, ldx,
+                             nr, sm.ybuf, sm.part, sm.zbuf);
         return;
     }
-    // entries [(q+2)B, n) in chunks of CHUNK
-    const int64_t e0 = (int64_t)(q + 2) * B + (int64_t)(blockIdx.x - 1) * CHUNK;
-    block_gemv(A, lda, qoff, e0, 64, B, ys, red, n);
-    if (t < CHUNK && e0 + t < n) x[e0 + t] += red[0][t];
+    // entries [(q+2)B, n) in chunks of CH
+    const int64_t e0 = (int64_t)(q + 2) * B + (int64_t)(blockIdx.x - 1) * CH;
+    block_gemv<NR, 1>(A, lda, qoff, e0, CH / 4, B, sm.vs, MAXB, nr, sm.red, n, X, ldx);
 }
 
 // Backward step q (q = N .. 1): apply solved block q (if q < N), solve q-1.
+template <int NR>
 __global__ __launch_bounds__(NT) void gesl_bwd_kernel(const float* __restrict__ A, int64_t lda,
                                                       int B, int q, int N,
                                                       const float* __restrict__ inv,
-                                                      int64_t inv_stride, float* __restrict__ x) {
-    __shared__ float ybuf[512];
-    __shared__ float part[8][32];
-    __shared__ float xs[512];
-    __shared__ float red[NT / 16][4 * 64];
+                                                      int64_t inv_stride, float* __restrict__ X,
+                                                      int64_t ldx, int nr) {
+    constexpr int MAXB = gesl_cfg<NR>::MAXB, CH = gesl_cfg<NR>::CH;
+    __shared__ gesl_smem<NR> sm;
     const int t = threadIdx.x;
     const int NQ = B / 32;
     const int64_t qoff = (int64_t)q * B;
     if (q < N) {
-        for (int e = t; e < B; e += NT) xs[e] = x[qoff + e];
+        for (int idx = t; idx < nr * B; idx += NT) {
+            const int r = idx / B, e = idx % B;
+            sm.vs[r * MAXB + e] = X[r * ldx + qoff + e];
+        }
         __syncthreads();
     }
     if (blockIdx.x == 0) {
         const int64_t Q = (int64_t)(q - 1) * B;
         if (q < N) {
-            for (int e0 = 0; e0 < B; e0 += 256) {
-                const int ncols = min(64, (B - e0) / 4);
-                block_gemv(A, lda, qoff, Q + e0, ncols, B, xs, red, Q + B);
-                if (t < 4 * ncols) x[Q + e0 + t] -= red[0][t];
-                __syncthreads();
+            for (int e0 = 0; e0 < B; e0 += 4 * NT / 2) {
+                const int ncols = min(NT / 2, (B - e0) / 4);
+                block_gemv<NR, -1>(A, lda, qoff, Q + e0, ncols, B, sm.vs, MAXB, nr, sm.red, Q + B,
+                                   X, ldx);
             }
+            __syncthreads();
         }
-        diag_solve<false>(A, lda, Q, B, inv + (int64_t)(q - 1) * inv_stride + 2 * NQ * 1024, x,
-                          ybuf, part);
+        diag_solve<NR, false>(A, lda, Q, B, inv + (int64_t)(q - 1) * inv_stride + 2 * NQ * 1024, X,
+                              ldx, nr, sm.ybuf, sm.part, sm.zbuf);
         return;
     }
     // entries [0, (q-1)B)
-    const int64_t e0 = (int64_t)(blockIdx.x - 1) * CHUNK;
+    const int64_t e0 = (int64_t)(blockIdx.x - 1) * CH;
     const int64_t lim = (int64_t)(q - 1) * B;
-    block_gemv(A, lda, qoff, e0, 64, B, xs, red, lim);
-    if (t < CHUNK && e0 + t < lim) x[e0 + t] -= red[0][t];
+    block_gemv<NR, -1>(A, lda, qoff, e0, CH / 4, B, sm.vs, MAXB, nr, sm.red, lim, X, ldx);
+}
+
+template <int NR>
+hipError_t gesl_sweeps(const float* A, int64_t lda, int64_t n, int B, const float* inv,
+                       int64_t inv_stride, float* X, int64_t ldx, int nr, hipStream_t s) {
+    constexpr int CH = gesl_cfg<NR>::CH;
+    if (B > gesl_cfg<NR>::MAXB) return hipErrorInvalidValue;
+    const int N = (int)(n / B);
+    for (int q = -1; q <= N - 2; ++q) {
+        const int64_t rest = n - (int64_t)(q + 2) * B;
+        const int grid = 1 + (q >= 0 ? (int)((rest + CH - 1) / CH) : 0);
+        gesl_fwd_kernel<NR><<<grid, NT, 0, s>>>(A, lda, n, B, q, inv, inv_stride, X, ldx, nr);
+    }
+    for (int q = N; q >= 1; --q) {
+        const int64_t rest = (int64_t)(q - 1) * B;
+        const int grid = 1 + (q < N ? (int)((rest + CH - 1) / CH) : 0);
+        gesl_bwd_kernel<NR><<<grid, NT, 0, s>>>(A, lda, B, q, N, inv, inv_stride, X, ldx, nr);
+    }
+    return hipGetLastError();
 }
 
 }  // namespace
 
+hipError_t launch_gesl_multi(const float* A, int64_t lda, int64_t n, int B, const float* inv,
+                             int64_t inv_stride, float* X, int64_t ldx, int nrhs, hipStream_t s) {
+    if (B > 512 || B < 32 || B % 32 || n % B || nrhs < 1 || (nrhs > 1 && (ldx < n || ldx % 4)))
+        return hipErrorInvalidValue;
+    for (int r0 = 0; r0 < nrhs; r0 += 16) {
+        const int nr = std::min(16, nrhs - r0);
+        float* Xr = X + (int64_t)r0 * ldx;
+        hipError_t e;
+        if (nr == 1)
+            e = gesl_sweeps<1>(A, lda, n, B, inv, inv_stride, Xr, ldx, 1, s);
+        else if (nr <= 4)
+            e = gesl_sweeps<4>(A, lda, n, B, inv, inv_stride, Xr, ldx, nr, s);
+        else
+            e = gesl_sweeps<16>(A, lda, n, B, inv, inv_stride, Xr, ldx, nr, s);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
 hipError_t launch_gesl(const float* A, int64_t lda, int64_t n, int B, const float* inv,
                        int64_t inv_stride, float* x, hipStream_t s) {
-    if (B > 512 || B % 32) return hipErrorInvalidValue;
-    const int N = (int)(n / B);
-    for (int q = -1; q <= N - 2; ++q) {
-        const int64_t rest = n - (int64_t)(q + 2) * B;
-        const int grid = 1 + (q >= 0 ? (int)((rest + CHUNK - 1) / CHUNK) : 0);
-        gesl_fwd_kernel<<<grid, NT, 0, s>>>(A, lda, n, B, q, inv, inv_stride, x);
-    }
-    for (int q = N; q >= 1; --q) {
-        const int64_t rest = (int64_t)(q - 1) * B;
-        const int grid = 1 + (q < N ? (int)((rest + CHUNK - 1) / CHUNK) : 0);
-        gesl_bwd_kernel<<<grid, NT, 0, s>>>(A, lda, B, q, N, inv, inv_stride, x);
-    }
-    return hipGetLastError();
+    return launch_gesl_multi(A, lda, n, B, inv, inv_stride, x, n, 1, s);
 }

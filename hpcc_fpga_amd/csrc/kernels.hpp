@@ -13,10 +13,13 @@ hipError_t launch_prep(const float* A, int64_t lda, int64_t off, int B, float* D
 hipError_t launch_prep_all(const float* A, int64_t lda, int B, int nblocks, float* inv,
                            int64_t inv_stride, hipStream_t s);
 // lp_prec 0: FP32 only; 1/2: also write BF16/FP16 k-contiguous copies of the
-// top strip (lpT[c][k]) and left strip (lpL[r][k]), leading dimension ldlp.
+// top strip (lpT[c][k]) and left strip (lpL[r][k]), leading dimension ldlp;
+// 3: FP8 E4M3 copies with the scale of vectors 32 g .. 32 g + 31 at scT[g] /
+// scL[g] (vectors counted from the strip start).
 hipError_t launch_trsm(float* A, int64_t lda, int64_t off, int64_t m, int B, const float* DT,
                        const float* inv, float* Lt, int64_t ldt, hipStream_t s, int lp_prec = 0,
-                       void* lpT = nullptr, void* lpL = nullptr, int64_t ldlp = 0);
+                       void* lpT = nullptr, void* lpL = nullptr, int64_t ldlp = 0,
+                       float* scT = nullptr, float* scL = nullptr);
 
 // trailing update (gemm.hip)
 hipError_t launch_gemm_update_f32(float* C, int64_t ldc, const float* Lt, int64_t ldl,
@@ -28,9 +31,21 @@ hipError_t launch_gemm_update_lp(int prec, float* C, int64_t ldc, const void* L,
                                  int64_t l_slot, const void* Tt, int64_t t_ld, int64_t t_slot,
                                  int M, int N, int K, int Bblk, hipStream_t s);
 
+// FP8 trailing update (gemm_lp.hip): C[r][c] += sum over the K / Bblk steps p
+// of scL[p][r/32] * scT[p][c/32] * sum_k L[r][k] T[c][k]  (E4M3 codes, FP32
+// accumulation); step p's scales at scL + p * sc_slot, scT + p * sc_slot
+hipError_t launch_gemm_update_fp8(float* C, int64_t ldc, const void* L, int64_t l_ld,
+                                  int64_t l_slot, const void* Tt, int64_t t_ld, int64_t t_slot,
+                                  const float* scL, const float* scT, int64_t sc_slot, int M,
+                                  int N, int K, int Bblk, hipStream_t s);
+
 // substitution (gesl.hip)
 hipError_t launch_gesl(const float* A, int64_t lda, int64_t n, int B, const float* inv,
                        int64_t inv_stride, float* x, hipStream_t s);
+// nrhs right-hand sides, column r at X + r * ldx (ldx a multiple of 4, >= n);
+// blocks up to 256 for more than 4 right-hand sides
+hipError_t launch_gesl_multi(const float* A, int64_t lda, int64_t n, int B, const float* inv,
+                             int64_t inv_stride, float* X, int64_t ldx, int nrhs, hipStream_t s);
 
 // generation and residuals (gen.hip)
 hipError_t launch_generate(float* A, int64_t lda, int64_t n, int mode, uint64_t seed, double d,
@@ -47,6 +62,25 @@ hipError_t launch_residual_f64x(const float* A, int64_t lda, int64_t n, const do
 hipError_t launch_d2f_scaled(const double* r, float* x, int64_t n, double scale, hipStream_t s);
 hipError_t launch_axpy_f2d(double* x, const float* d, int64_t n, double scale, int assign,
                            hipStream_t s);
+
+// several right-hand sides (gen.hip), nrhs <= 16, column r at offset r * ld:
+// R = B - M X in FP64 with out[4 r + 0..3] = (||r||, ||M||, ||x||, ||b||)_inf;
+// work holds residual_multi_work(n, nrhs) doubles
+int64_t residual_multi_work(int64_t n, int nrhs);
+hipError_t launch_residual_multi(const float* A, int64_t lda, int64_t n, const double* X,
+                                 int64_t ldx, const double* Bv, int64_t ldb, int nrhs,
+                                 double* work, double* R, int64_t ldr, double* out,
+                                 hipStream_t s);
+// Xs[r] = (float)(R[r] * scale[r]);  X[r] = scale[r] * D[r] (+ X[r])
+hipError_t launch_d2f_multi(const double* R, int64_t ldr, float* Xs, int64_t ldxs, int64_t n,
+                            int nrhs, const double* scale, hipStream_t s);
+hipError_t launch_axpy_multi(double* X, int64_t ldx, const float* D, int64_t ldd, int64_t n,
+                             int nrhs, const double* scale, int assign, hipStream_t s);
+
+// identity extension of an n x n system to npad x npad; fill (kernels, no
+// memset nodes) (gen.hip)
+hipError_t launch_pad_identity(float* A, int64_t lda, int64_t n, int64_t npad, hipStream_t s);
+hipError_t launch_fill_f64(double* p, int64_t count, double v, hipStream_t s);
 
 // batched solves (batch.hip)
 struct BatchSystem {

@@ -14,6 +14,11 @@
 // as the reference's left_update writes its transposed copy
 // (hpl_torus_PCIE.cl:697-708).
 //
+// Mixed precision: the solved strips can also be written as BF16/FP16 copies
+// (k-contiguous) or as FP8 E4M3 copies with one power-of-two scale per block
+// of 32 vectors x B (the block's amax maps to at most 448, so the scaling is
+// exact and nothing saturates).
+//
 // Blocking: rows are taken 32 at a time.  For block q the contribution of the
 // already solved rows is a 32 x 32q by 32q x 32 product on MFMA
 // (v_mfma_f32_32x32x2_f32), and the 32 x 32 diagonal system is applied as its
@@ -115,7 +120,10 @@ __global__ __launch_bounds__(64) void trsm_kernel(float* __restrict__ A, int64_t
                                                   const float* __restrict__ inv,
                                                   float* __restrict__ Lt, int64_t ldt,
                                                   int ngroups_top, LP* __restrict__ lpT,
-                                                  LP* __restrict__ lpL, int64_t ldlp) {
+                                                  LP* __restrict__ lpL, int64_t ldlp,
+                                                  float* __restrict__ scT,
+                                                  float* __restrict__ scL) {
+    constexpr bool kFp8 = std::is_same<LP, fp8e4m3>::value;
     constexpr int NQ = B / 32;
     __shared__ float Y[B][32];
     __shared__ float SZ[32][33];  // staging tile of the left strip, then z_q
@@ -206,7 +214,7 @@ __global__ __launch_bounds__(64) void trsm_kernel(float* __restrict__ A, int64_t
 #pragma unroll
         for (int r = 0; r < 16; ++r) Y[q * 32 + acc_row(r, lane)][lr] = y[r];
         // ---- low-precision copy: registers 4g..4g+3 are 4 consecutive k
-        if constexpr (!std::is_same<LP, float>::value) {
+        if constexpr (!std::is_same<LP, float>::value && !kFp8) {
             LP* dst = (top ? lpT : lpL) + (v0 + lr) * ldlp + q * 32 + 4 * lh;
 #pragma unroll
             for (int g = 0; g < 4; ++g) {
@@ -240,6 +248,32 @@ __global__ __launch_bounds__(64) void trsm_kernel(float* __restrict__ A, int64_t
         }
         wave_lds_sync();
     }
+    // ---- FP8 copy with one power-of-two scale per block of 32 vectors x B:
+    // lane (lr, lh) quantises vector lr, k in [lh B/2, (lh+1) B/2)
+    if constexpr (kFp8) {
+        constexpr int KH = B / 2;
+        float am = 0.0f;
+#pragma unroll 8
+        for (int k = lh * KH; k < (lh + 1) * KH; ++k) am = fmaxf(am, fabsf(Y[k][lr]));
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) am = fmaxf(am, __shfl_xor(am, o));
+        const int e = fp8_scale_exp(am);
+        const float inv = exp2i(-e);
+        uint8_t* dst = reinterpret_cast<uint8_t*>(top ? lpT : lpL) + (v0 + lr) * ldlp + lh * KH;
+#pragma unroll
+        for (int c = 0; c < KH; c += 16) {
+            uint32_t w[4];
+#pragma unroll
+            for (int q4 = 0; q4 < 4; ++q4) {
+                const int k = lh * KH + c + 4 * q4;
+                w[q4] = fp8_e4m3_code(Y[k][lr] * inv) | (fp8_e4m3_code(Y[k + 1][lr] * inv) << 8) |
+                        (fp8_e4m3_code(Y[k + 2][lr] * inv) << 16) |
+                        (fp8_e4m3_code(Y[k + 3][lr] * inv) << 24);
+            }
+            *reinterpret_cast<uint4*>(dst + c) = make_uint4(w[0], w[1], w[2], w[3]);
+        }
+        if (lane == 0) (top ? scT : scL)[v0 / 32] = exp2i(e);
+    }
 }
 
 }  // namespace
@@ -263,13 +297,14 @@ hipError_t launch_prep_all(const float* A, int64_t lda, int B, int nblocks, floa
 template <typename LP>
 static hipError_t launch_trsm_t(float* A, int64_t lda, int64_t off, int64_t m, int B,
                                 const float* DT, const float* inv, float* Lt, int64_t ldt,
-                                LP* lpT, LP* lpL, int64_t ldlp, hipStream_t s) {
+                                LP* lpT, LP* lpL, int64_t ldlp, hipStream_t s,
+                                float* scT = nullptr, float* scL = nullptr) {
     const int groups = (int)(m / 32);
     dim3 grid(2 * groups);
 #define HPL_TRSM_CASE(BB)                                                                     \
     case BB:                                                                                  \
         trsm_kernel<BB, LP><<<grid, 64, 0, s>>>(A, lda, off, m, DT, inv, Lt, ldt, groups, lpT, \
-                                                lpL, ldlp);                                   \
+                                                lpL, ldlp, scT, scL);                         \
         break;
     switch (B) {
         HPL_TRSM_CASE(32)
@@ -285,7 +320,7 @@ static hipError_t launch_trsm_t(float* A, int64_t lda, int64_t off, int64_t m, i
 
 hipError_t launch_trsm(float* A, int64_t lda, int64_t off, int64_t m, int B, const float* DT,
                        const float* inv, float* Lt, int64_t ldt, hipStream_t s, int lp_prec,
-                       void* lpT, void* lpL, int64_t ldlp) {
+                       void* lpT, void* lpL, int64_t ldlp, float* scT, float* scL) {
     if (m <= 0) return hipSuccess;
     if (m % 32) return hipErrorInvalidValue;
     switch (lp_prec) {
@@ -299,6 +334,11 @@ hipError_t launch_trsm(float* A, int64_t lda, int64_t off, int64_t m, int B, con
             return launch_trsm_t<_Float16>(A, lda, off, m, B, DT, inv, Lt, ldt,
                                            static_cast<_Float16*>(lpT),
                                            static_cast<_Float16*>(lpL), ldlp, s);
+        case 3:
+            if (!scT || !scL || ldlp % 16) return hipErrorInvalidValue;
+            return launch_trsm_t<fp8e4m3>(A, lda, off, m, B, DT, inv, Lt, ldt,
+                                          static_cast<fp8e4m3*>(lpT), static_cast<fp8e4m3*>(lpL),
+                                          ldlp, s, scT, scL);
         default:
             return hipErrorInvalidValue;
     }

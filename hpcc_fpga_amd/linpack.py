@@ -51,6 +51,33 @@ class _BatchStats(C.Structure):
                 ("failed", C.c_int), ("zero_pivots", C.c_int)]
 
 
+class _ServeConfig(C.Structure):
+    _fields_ = [("device", C.c_int), ("slots", C.c_int), ("hbm_budget", C.c_uint64),
+                ("prec", C.c_int), ("max_iter", C.c_int), ("tol_per_n", C.c_double),
+                ("max_n", C.c_uint64), ("max_nrhs", C.c_int)]
+
+
+class _Request(C.Structure):
+    _fields_ = [("n", C.c_uint64), ("nrhs", C.c_int), ("a", C.c_void_p), ("lda", C.c_uint64),
+                ("seed", C.c_uint64), ("factor", C.c_uint64), ("keep", C.c_int),
+                ("b", C.c_void_p), ("ldb", C.c_uint64), ("x", C.c_void_p), ("ldx", C.c_uint64)]
+
+
+class _Response(C.Structure):
+    _fields_ = [("status", C.c_int), ("factor", C.c_uint64), ("reused", C.c_int),
+                ("iterations", C.c_int), ("backward_error", C.c_double), ("queue_s", C.c_double),
+                ("service_s", C.c_double), ("latency_s", C.c_double), ("block", C.c_uint32),
+                ("npad", C.c_uint64)]
+
+
+class _ServeStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("failed", C.c_uint64),
+                ("queued", C.c_int), ("in_flight", C.c_int), ("max_in_flight", C.c_int),
+                ("resident_factors", C.c_uint64), ("factor_bytes_in_use", C.c_uint64),
+                ("peak_factor_bytes", C.c_uint64), ("factor_budget_bytes", C.c_uint64),
+                ("workspace_bytes", C.c_uint64)]
+
+
 class _Timing(C.Structure):
     _fields_ = [("gefa_s", C.c_double), ("gesl_s", C.c_double), ("h2d_s", C.c_double),
                 ("d2h_s", C.c_double)]
@@ -99,6 +126,15 @@ def load_library():
         "hpl_solve_ir": ([P, u32, i32, i32, dbl, C.POINTER(_IRResult)], i32),
         "hpl_download_solution64": ([P, P], i32),
         "hpl_upload_rhs64": ([P, P], i32),
+        "hpl_solve_rhs64": ([P, i32, P, u64, P, u64, i32, dbl, P], i32),
+        "hpl_serve_config_default": ([C.POINTER(_ServeConfig)], None),
+        "hpl_serve_create": ([C.POINTER(_ServeConfig), C.POINTER(P)], i32),
+        "hpl_serve_destroy": ([P], None),
+        "hpl_serve_last_error": ([P], C.c_char_p),
+        "hpl_serve_submit": ([P, C.POINTER(_Request), C.POINTER(u64)], i32),
+        "hpl_serve_wait": ([P, u64, C.POINTER(_Response)], i32),
+        "hpl_serve_release": ([P, u64], i32),
+        "hpl_serve_stats_get": ([P, C.POINTER(_ServeStats)], i32),
         "hpl_batch_create": ([P, P, i32, C.POINTER(P)], i32),
         "hpl_batch_destroy": ([P], None),
         "hpl_batch_last_error": ([P], C.c_char_p),
@@ -256,6 +292,26 @@ class Context:
         b = np.ascontiguousarray(b, np.float64)
         self._check(self._L.hpl_upload_rhs64(self._c, b.ctypes.data))
 
+    def solve_rhs64(self, b: np.ndarray, max_iter: int = 50, tol: float = None,
+                    check: bool = True):
+        """Solve and refine further right-hand sides (n or n x nrhs, FP64) with
+        the resident factor; returns (x, [per-column result dicts])."""
+        b2 = np.asarray(b, np.float64)
+        one = b2.ndim == 1
+        cols = np.ascontiguousarray((b2[:, None] if one else b2).T)  # nrhs x n, column-major
+        nrhs = cols.shape[0]
+        x = np.empty_like(cols)
+        if tol is None:
+            tol = 1e-14 * self.n
+        res = (_IRResult * nrhs)()
+        st = self._L.hpl_solve_rhs64(self._c, nrhs, cols.ctypes.data, self.n, x.ctypes.data,
+                                     self.n, max_iter, tol, res)
+        if st != HPL_SUCCESS and (check or st != 7):
+            self._check(st)
+        out = [{f: getattr(r, f) for f, _ in _IRResult._fields_} for r in res]
+        x = x.T
+        return (x[:, 0].copy() if one else np.ascontiguousarray(x)), out
+
     def set_profiling(self, enable: bool = True):
         self._check(self._L.hpl_set_profiling(self._c, int(enable)))
 
@@ -326,6 +382,101 @@ class Batch:
         b = np.empty(npad, np.float64)
         self._check(self._L.hpl_batch_get_system(self._b, s, a.ctypes.data, b.ctypes.data))
         return a, b
+
+
+class Ticket:
+    """A submitted request; holds its host buffers until it is waited for."""
+
+    def __init__(self, server, ticket, n, nrhs, a, b, x):
+        self.server, self.ticket, self.n, self.nrhs = server, ticket, n, nrhs
+        self._keep = (a, b)
+        self.x = x  # nrhs x n (column j of the solution is row j)
+        self.done = False
+
+
+class Server:
+    """Request loop over one device (see hpl_serve_* in include/hpl_hip.h).
+
+    submit() returns at once with a Ticket; wait(ticket) blocks until that
+    request is solved and returns (x, response).  Both release the GIL inside
+    the library, so one thread can submit while another waits."""
+
+    def __init__(self, device: int = 0, slots: int = 4, hbm_budget: int = 200 * 10 ** 9,
+                 prec: str = "fp8", max_iter: int = 30, tol_per_n: float = 1e-14,
+                 max_n: int = 8192, max_nrhs: int = 16):
+        self._L = load_library()
+        cfg = _ServeConfig()
+        self._L.hpl_serve_config_default(C.byref(cfg))
+        cfg.device, cfg.slots, cfg.hbm_budget = device, slots, int(hbm_budget)
+        cfg.prec, cfg.max_iter, cfg.tol_per_n = PRECISIONS[prec], max_iter, tol_per_n
+        cfg.max_n, cfg.max_nrhs = max_n, max_nrhs
+        self._s = C.c_void_p()
+        st = self._L.hpl_serve_create(C.byref(cfg), C.byref(self._s))
+        if st != HPL_SUCCESS:
+            raise HplError(st, self._L.hpl_serve_last_error(None).decode())
+
+    def _check(self, st):
+        if st != HPL_SUCCESS:
+            raise HplError(st, self._L.hpl_serve_last_error(self._s).decode())
+
+    def submit(self, b: np.ndarray, a: np.ndarray | None = None, seed: int | None = None,
+               factor: int = 0, keep: bool = False) -> Ticket:
+        """b: n or n x nrhs FP64.  The system is `a` (row-major float32, M =
+        a.T), or generated on the device from `seed`, or the resident
+        `factor`."""
+        b2 = np.asarray(b, np.float64)
+        cols = np.ascontiguousarray((b2[:, None] if b2.ndim == 1 else b2).T)
+        nrhs, n = cols.shape
+        x = np.empty_like(cols)
+        r = _Request()
+        r.n, r.nrhs = n, nrhs
+        if a is not None:
+            a = _f32(np.ascontiguousarray(a, np.float32))
+            r.a, r.lda = a.ctypes.data, a.shape[1]
+        r.seed = 0 if seed is None else int(seed)
+        r.factor, r.keep = int(factor), int(keep)
+        r.b, r.ldb, r.x, r.ldx = cols.ctypes.data, n, x.ctypes.data, n
+        t = C.c_uint64()
+        self._check(self._L.hpl_serve_submit(self._s, C.byref(r), C.byref(t)))
+        return Ticket(self, t.value, n, nrhs, a, cols, x)
+
+    def wait(self, ticket: Ticket, check: bool = True):
+        """(x as n or n x nrhs, response dict)."""
+        resp = _Response()
+        st = self._L.hpl_serve_wait(self._s, ticket.ticket, C.byref(resp))
+        ticket.done = True
+        out = {f: getattr(resp, f) for f, _ in _Response._fields_}
+        if st != HPL_SUCCESS:
+            out["error"] = self._L.hpl_serve_last_error(self._s).decode()
+            if check:
+                raise HplError(st, out["error"])
+        x = ticket.x.T
+        return (x[:, 0].copy() if ticket.nrhs == 1 else np.ascontiguousarray(x)), out
+
+    def release(self, factor: int):
+        self._check(self._L.hpl_serve_release(self._s, int(factor)))
+
+    def stats(self) -> dict:
+        s = _ServeStats()
+        self._check(self._L.hpl_serve_stats_get(self._s, C.byref(s)))
+        return {f: getattr(s, f) for f, _ in _ServeStats._fields_}
+
+    def close(self):
+        if self._s:
+            self._L.hpl_serve_destroy(self._s)
+            self._s = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def gefa(a: np.ndarray, block: int, device: int = 0) -> np.ndarray:

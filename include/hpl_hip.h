@@ -51,7 +51,8 @@ typedef enum hpl_prec {
     HPL_PREC_FP32 = 0,
     HPL_PREC_BF16 = 1,
     HPL_PREC_FP16 = 2,
-    HPL_PREC_FP8 = 3 /* E4M3 with one FP32 scale per B x B operand block */
+    HPL_PREC_FP8 = 3 /* OCP E4M3 with one power-of-two FP32 scale per block of
+                        32 rows (columns) x B of each panel operand */
 } hpl_prec;
 
 /* Wall times of the last call, measured with HIP events on the library's
@@ -150,6 +151,13 @@ typedef struct hpl_ir_result {
 } hpl_ir_result;
 hpl_status hpl_solve_ir(hpl_ctx* ctx, uint32_t block, hpl_prec prec, int max_iter, double tol,
                         hpl_ir_result* result);
+/* Solve nrhs (1..16) further right-hand sides with the resident factor (of
+ * hpl_solve_ir, or of hpl_gefa_dev on a kept original) and refine each column
+ * as above: b and x are n x nrhs FP64, column j at b + j*ldb / x + j*ldx;
+ * res[j] receives column j's result (gefa_s and total_s 0).  No reference
+ * counterpart: the reference solves one right-hand side per factorisation. */
+hpl_status hpl_solve_rhs64(hpl_ctx* ctx, int nrhs, const double* b, uint64_t ldb, double* x,
+                           uint64_t ldx, int max_iter, double tol, hpl_ir_result* res);
 /* The refined FP64 solution of the last hpl_solve_ir. */
 hpl_status hpl_download_solution64(hpl_ctx* ctx, double* x);
 /* Replace the resident right-hand side by an FP64 vector (FP32 copy derived). */
@@ -192,6 +200,85 @@ hpl_status hpl_batch_get(hpl_batch* batch, int s, double* x, int* iterations, do
 /* The padded matrix (npad x npad, npad = n rounded up to 32) and right-hand
  * side (npad doubles) of system s as stored on the device. */
 hpl_status hpl_batch_get_system(hpl_batch* batch, int s, float* a, double* b);
+
+/* ---- serving: a request loop over one device -----------------------------
+ * No reference counterpart (the reference runs one benchmark matrix per
+ * process).  A server admits solves of varying order n and right-hand-side
+ * count nrhs as they arrive.  `slots` worker threads each run one solve at a
+ * time on their own HIP stream, so requests that arrive while others run are
+ * admitted as soon as a slot and memory free up, and their device work
+ * overlaps (continuous batching).  Admission is strictly in arrival order and
+ * by memory: the server holds `hbm_budget` bytes of device memory, of which
+ * the slot workspaces are reserved at creation and the rest holds the factors
+ * (original + factor + inverses, about 8 n^2 bytes each).
+ *
+ * A new system is factorised with the trailing update in `prec` (FP32
+ * accumulation, FP32 panel and solves), then every right-hand side is refined
+ * in FP64 to a backward error <= tol_per_n * n.  A factor can be kept resident
+ * (`keep`) and later requests solve new right-hand sides with it (`factor`)
+ * without refactorising.  The block size and padding depend on n alone, so a
+ * request's result does not depend on what else is running.
+ *
+ * Host buffers of a request (a, b, x) must stay valid until hpl_serve_wait
+ * returns for its ticket; each ticket is waited for once. */
+typedef struct hpl_server hpl_server;
+typedef struct hpl_serve_config {
+    int device;          /* HIP device                                       */
+    int slots;           /* concurrent solves, 1..8                          */
+    uint64_t hbm_budget; /* device bytes held by the server                  */
+    hpl_prec prec;       /* trailing-update precision of new factorisations */
+    int max_iter;        /* refinement corrections per right-hand side       */
+    double tol_per_n;    /* backward-error target per unit of n              */
+    uint64_t max_n;      /* largest order admitted                           */
+    int max_nrhs;        /* largest right-hand-side count, 1..16             */
+} hpl_serve_config;
+typedef struct hpl_request {
+    uint64_t n;       /* order                                                 */
+    int nrhs;         /* right-hand sides                                      */
+    const float* a;   /* row-major n x n (M = a^T), or NULL                    */
+    uint64_t lda;
+    uint64_t seed;    /* a == NULL, factor == 0: generate M on the device:
+                         U[-0.5,0.5) off the diagonal, M(i,i) += n (the
+                         counter-based generator of hpl_generate mode 0)       */
+    uint64_t factor;  /* non-zero: solve with this resident factor            */
+    int keep;         /* keep the new factor resident (handle in the response)*/
+    const double* b;  /* n x nrhs, column j at b + j*ldb                       */
+    uint64_t ldb;
+    double* x;        /* n x nrhs solution, column j at x + j*ldx              */
+    uint64_t ldx;
+} hpl_request;
+typedef struct hpl_response {
+    hpl_status status;
+    uint64_t factor;       /* resident handle (kept or reused), else 0        */
+    int reused;            /* 1: solved with a resident factor                */
+    int iterations;        /* most refinement corrections over the columns    */
+    double backward_error; /* largest backward error over the columns         */
+    double queue_s;        /* submit -> admitted                              */
+    double service_s;      /* admitted -> done                                */
+    double latency_s;      /* submit -> done                                  */
+    uint32_t block;        /* block size used                                 */
+    uint64_t npad;         /* order after identity padding to a block multiple*/
+} hpl_response;
+typedef struct hpl_serve_stats {
+    uint64_t submitted, completed, failed;
+    int queued, in_flight, max_in_flight;
+    uint64_t resident_factors;
+    uint64_t factor_bytes_in_use, peak_factor_bytes, factor_budget_bytes;
+    uint64_t workspace_bytes; /* slot workspaces reserved at creation */
+} hpl_serve_stats;
+/* Defaults: device 0, 4 slots, 200 GB, FP8, 30 corrections, 1e-14, n <= 8192,
+ * nrhs <= 16. */
+void hpl_serve_config_default(hpl_serve_config* cfg);
+hpl_status hpl_serve_create(const hpl_serve_config* cfg, hpl_server** server);
+/* Finishes the admitted and queued requests, then frees everything. */
+void hpl_serve_destroy(hpl_server* server);
+const char* hpl_serve_last_error(const hpl_server* server);
+hpl_status hpl_serve_submit(hpl_server* server, const hpl_request* req, uint64_t* ticket);
+/* Block until the request is done; returns its status (also in resp). */
+hpl_status hpl_serve_wait(hpl_server* server, uint64_t ticket, hpl_response* resp);
+/* Give up a resident factor (freed once no queued request uses it). */
+hpl_status hpl_serve_release(hpl_server* server, uint64_t factor);
+hpl_status hpl_serve_stats_get(hpl_server* server, hpl_serve_stats* stats);
 
 /* ---- kernel timing for roofline accounting -------------------------------
  * When enabled, every trailing-update GEMM launch is bracketed by HIP events

@@ -18,6 +18,7 @@
 // Build: oracle/Makefile (g++ -O3 -ffp-contract=off: no FMA contraction, so
 // the float loops round exactly like the reference compiled for plain x86-64).
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdlib>
@@ -295,6 +296,94 @@ void orc_gefa_blocked(float* a, int64_t n, int64_t lda, int64_t B) {
                 orc_inner_update(a + i * B * lda + j * B, lda,
                                  a + i * B * lda + k * B, lda,
                                  a + k * B * lda + j * B, lda, B);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Mixed-precision trailing update (no reference counterpart; the reference's
+// inner_update_mm is FP32, hpl_torus_PCIE.cl:711-865).  Rounding of a value to
+// a binary format with `frac` fraction bits, minimum normal exponent `emin`
+// and largest finite value `maxv`, round to nearest with ties to even (the
+// default rounding mode of nearbyint), subnormals included; values beyond
+// maxv saturate (only FP8 can reach it, and its block scale prevents that).
+static double round_format(double x, int frac, int emin, double maxv) {
+    if (x == 0.0 || !std::isfinite(x)) return x;
+    int e = std::ilogb(x);
+    if (e < emin) e = emin;
+    const double q = std::ldexp(std::nearbyint(std::ldexp(x, frac - e)), e - frac);
+    return std::fabs(q) > maxv ? std::copysign(maxv, q) : q;
+}
+
+double orc_round_lp(double x, int prec) {
+    switch (prec) {
+        case 1: return round_format(x, 7, -126, 3.3895313892515355e38);  // bfloat16
+        case 2: return round_format(x, 10, -14, 65504.0);                // IEEE binary16
+        case 3: return round_format(x, 3, -6, 448.0);                    // OCP E4M3
+        default: return x;
+    }
+}
+
+// Power-of-two block scale of FP8: 2^e with e the smallest integer such that
+// amax <= 448 * 2^e (1 for an all-zero block).
+double orc_fp8_scale(double amax) {
+    if (!(amax > 0.0)) return 1.0;
+    int e = (int)std::floor(std::log2(amax / 448.0));
+    while (amax > 448.0 * std::ldexp(1.0, e)) ++e;
+    while (amax <= 448.0 * std::ldexp(1.0, e - 1)) --e;
+    return std::ldexp(1.0, e);
+}
+
+// Quantise-dequantise `count` vectors of length B (vector v at x + v*stride_v,
+// element k at + k*stride_k) as the trailing update sees them: BF16/FP16
+// rounding, or FP8 with one scale per group of 32 consecutive vectors.
+static void quantise_vectors(const float* x, int64_t stride_v, int64_t stride_k, int64_t count,
+                             int64_t B, int prec, double* out /* count x B */) {
+    for (int64_t g = 0; g < count; g += 32) {
+        const int64_t ge = std::min<int64_t>(count, g + 32);
+        double scale = 1.0;
+        if (prec == 3) {
+            double amax = 0.0;
+            for (int64_t v = g; v < ge; v++)
+                for (int64_t k = 0; k < B; k++)
+                    amax = std::max(amax, std::fabs((double)x[v * stride_v + k * stride_k]));
+            scale = orc_fp8_scale(amax);
+        }
+        for (int64_t v = g; v < ge; v++)
+            for (int64_t k = 0; k < B; k++)
+                out[v * B + k] =
+                    scale * orc_round_lp((double)x[v * stride_v + k * stride_k] / scale, prec);
+    }
+}
+
+// Blocked factorisation as orc_gefa_blocked, with the trailing update taken
+// from low-precision copies of the solved strips (prec 1 BF16, 2 FP16, 3 FP8)
+// and accumulated exactly (double) before rounding to FP32 once per step.
+void orc_gefa_blocked_lp(float* a, int64_t n, int64_t lda, int64_t B, int prec) {
+    const int64_t N = n / B;
+    std::vector<double> Lq, Tq;
+    for (int64_t k = 0; k < N; k++) {
+        float* d = a + k * B * lda + k * B;
+        orc_gefa_nopvt(d, B, lda);
+        for (int64_t j = k + 1; j < N; j++)
+            orc_top_update(a + k * B * lda + j * B, lda, d, lda, B);
+        for (int64_t i = k + 1; i < N; i++)
+            orc_left_update(a + i * B * lda + k * B, lda, d, lda, B);
+        const int64_t o = (k + 1) * B, m = n - o;
+        if (m <= 0) break;
+        // top strip: vectors = columns c (element k at row k*B + kk);
+        // left strip: vectors = rows r (element at column k*B + kk)
+        Tq.assign(m * B, 0.0);
+        Lq.assign(m * B, 0.0);
+        quantise_vectors(a + k * B * lda + o, 1, lda, m, B, prec, Tq.data());
+        quantise_vectors(a + o * lda + k * B, lda, 1, m, B, prec, Lq.data());
+        for (int64_t r = 0; r < m; r++)
+            for (int64_t c = 0; c < m; c++) {
+                double s = (double)a[(o + r) * lda + o + c];
+                const double* lr = &Lq[r * B];
+                const double* tc = &Tq[c * B];
+                for (int64_t kk = 0; kk < B; kk++) s += lr[kk] * tc[kk];
+                a[(o + r) * lda + o + c] = (float)s;
+            }
     }
 }
 

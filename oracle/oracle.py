@@ -56,6 +56,11 @@ def lib():
             fn.argtypes = [C.c_void_p, _i64, C.c_void_p, _i64, _i64]
         L.orc_inner_update.argtypes = [C.c_void_p, _i64, C.c_void_p, _i64, C.c_void_p, _i64, _i64]
         L.orc_gefa_blocked.argtypes = [_f32p, _i64, _i64, _i64]
+        L.orc_gefa_blocked_lp.argtypes = [_f32p, _i64, _i64, _i64, C.c_int]
+        L.orc_round_lp.argtypes = [C.c_double, C.c_int]
+        L.orc_round_lp.restype = C.c_double
+        L.orc_fp8_scale.argtypes = [C.c_double]
+        L.orc_fp8_scale.restype = C.c_double
         L.orc_residn_ref.argtypes = [_f32p, _i64, C.c_float, C.POINTER(C.c_double),
                                      C.POINTER(C.c_double)]
         L.orc_norms.argtypes = [_f32p, _i64, _i64, _f64p, _f64p, _f64p]
@@ -185,6 +190,25 @@ def gefa_blocked(A: np.ndarray, B: int) -> np.ndarray:
     a = np.ascontiguousarray(A, np.float32).copy()
     lib().orc_gefa_blocked(a, a.shape[0], a.shape[1], B)
     return a
+
+
+LP_PRECISIONS = {"bf16": 1, "fp16": 2, "fp8": 3}
+
+
+def gefa_blocked_lp(A: np.ndarray, B: int, prec: str) -> np.ndarray:
+    """Blocked factorisation whose trailing update uses BF16/FP16 copies or
+    FP8 E4M3 copies with per-(32 vectors x B) power-of-two scales."""
+    a = np.ascontiguousarray(A, np.float32).copy()
+    lib().orc_gefa_blocked_lp(a, a.shape[0], a.shape[1], B, LP_PRECISIONS[prec])
+    return a
+
+
+def round_lp(x: float, prec: str) -> float:
+    return float(lib().orc_round_lp(float(x), LP_PRECISIONS[prec]))
+
+
+def fp8_scale(amax: float) -> float:
+    return float(lib().orc_fp8_scale(float(amax)))
 
 
 # --------------------------------------------------------------------------- metrics

@@ -4,6 +4,11 @@ Bars:
   * BF16/FP16 factor vs the oracle's FP32 gefa_ref_nopvt factor: relative
     max difference <= 2e-2 (BF16, unit roundoff 2^-8) / 4e-3 (FP16, 2^-11) --
     a wrong MFMA operand layout gives O(1) differences;
+  * BF16/FP16/FP8 factor vs the oracle's emulation of the same quantised
+    update (gefa_blocked_lp): the RMS distance to the emulation is at most a
+    quarter of the emulation's own RMS distance to the FP32 factor, i.e. the
+    GPU's departure from FP32 is the modelled quantisation (rounding mode,
+    FP8 code and block scale) and not something else;
   * refinement: normwise backward error <= 1e-14 * n (the target of the
     mixed-precision configuration) and the refined x within 1e-10 (relative,
     inf-norm) of the FP64 oracle solution (gefa_ref_nopvt/gesl_ref_nopvt
@@ -32,7 +37,27 @@ def test_lowprec_factor_close_to_fp32(ctx, prec, tol):
     assert err <= tol, err
 
 
-@pytest.mark.parametrize("prec,d", [("bf16", 2.0), ("bf16", 1.1), ("fp16", 1.1)])
+@pytest.mark.parametrize("prec,n,B", [("bf16", 512, 64), ("fp16", 512, 64), ("fp8", 512, 64),
+                                      ("fp8", 768, 128)])
+def test_lowprec_factor_matches_quantised_emulation(ctx, prec, n, B):
+    print(f"quantised emulation {prec} n={n} B={B}", flush=True)
+    A, _, _, _ = O.generate_ref(n, seed=9)  # positive entries: coherent, large updates
+    ctx.keep_original(False)
+    ctx.upload(A)
+    ctx.gefa_dev(B, prec)
+    F, _ = ctx.download(matrix=True, solution=False)
+    emu = O.gefa_blocked_lp(A, B, prec).astype(np.float64)
+    f32 = O.gefa_blocked(A, B).astype(np.float64)
+    rms = lambda d: float(np.sqrt(np.mean(d * d)))  # noqa: E731
+    d_gpu = rms(F - emu)
+    d_model = rms(emu - f32)
+    print(f"  rms(gpu-emu)={d_gpu:.3e} rms(emu-fp32)={d_model:.3e}", flush=True)
+    assert d_model > 0
+    assert d_gpu <= 0.25 * d_model
+
+
+@pytest.mark.parametrize("prec,d", [("bf16", 2.0), ("bf16", 1.1), ("fp16", 1.1), ("fp8", 2.0),
+                                    ("fp8", 1.1)])
 def test_refinement_reaches_fp64_accuracy(ctx, prec, d):
     n, B = 2048, 256
     print(f"refinement {prec} d={d}", flush=True)
@@ -50,6 +75,30 @@ def test_refinement_reaches_fp64_accuracy(ctx, prec, d):
     # independent residual on the CPU
     be = O.backward_error(A, x, b64)
     assert be <= 1e-14 * n, be
+    ctx.keep_original(False)
+
+
+@pytest.mark.parametrize("prec,nrhs", [("bf16", 1), ("fp8", 3), ("fp8", 16), ("fp32", 5)])
+def test_further_right_hand_sides(ctx, prec, nrhs):
+    """hpl_solve_rhs64: new right-hand sides with the resident factor, each
+    refined to 1e-14 n and within 1e-10 of the FP64 oracle solution."""
+    n, B = 1024, 128
+    print(f"further rhs {prec} nrhs={nrhs}", flush=True)
+    ctx.keep_original(True)
+    ctx.generate(n, 1, 5, 1.5)
+    ctx.solve_ir(B, prec, max_iter=30)
+    rng = np.random.default_rng(nrhs)
+    Bm = rng.uniform(-1, 1, (n, nrhs))
+    X, res = ctx.solve_rhs64(Bm if nrhs > 1 else Bm[:, 0], max_iter=30)
+    X = X.reshape(n, nrhs)
+    A, _, _, _ = O.generate_sized(n, 5, 1, 1.5)
+    F = O.gefa_nopvt_f64(A.astype(np.float64))
+    for j in range(nrhs):
+        assert res[j]["converged"] == 1 and res[j]["backward_error"] <= 1e-14 * n
+        x_ref = O.gesl_nopvt_f64(F, Bm[:, j])
+        fwd = np.max(np.abs(X[:, j] - x_ref)) / np.max(np.abs(x_ref))
+        assert fwd <= 1e-10, (j, fwd)
+        assert O.backward_error(A, X[:, j], Bm[:, j]) <= 1e-14 * n
     ctx.keep_original(False)
 
 

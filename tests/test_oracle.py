@@ -250,3 +250,90 @@ def test_golden_vectors():
     As, bs, _, _ = O.generate_sized(64, 42, 0)
     assert np.array_equal(As, g["sized64_A"]) and np.array_equal(bs, g["sized64_b"])
     assert float(g["readme_normb"]) == normb
+
+
+# --------------------------------------------------------------------------- mixed precision
+def _bf16_bits(v):
+    """bfloat16 rounding by the usual bit trick (ties to even on the kept bits)."""
+    u = np.float32(v).view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return float(np.uint32(r).view(np.float32))
+
+
+def _e4m3_table():
+    vals = []
+    for e in range(16):
+        for m in range(8):
+            if e == 15 and m == 7:
+                continue  # NaN code
+            vals.append(m * 2.0 ** -9 if e == 0 else (1 + m / 8) * 2.0 ** (e - 7))
+    return np.array(vals)
+
+
+def test_low_precision_rounding_against_independent_definitions():
+    """round_lp: FP16 against numpy's float16 cast, BF16 against the bit
+    trick, FP8 E4M3 against brute-force nearest value of the code table (ties
+    to the even code)."""
+    rng = np.random.default_rng(0)
+    x = (rng.standard_normal(4000) * np.exp2(rng.integers(-20, 12, 4000))).astype(np.float32)
+    assert all(O.round_lp(v, "fp16") == float(np.float16(v)) for v in x)
+    assert all(O.round_lp(v, "bf16") == _bf16_bits(v) for v in x)
+    vals = _e4m3_table()
+    xs = rng.uniform(-448, 448, 4000) * np.exp2(rng.integers(-12, 1, 4000))
+    # exact midpoints between neighbouring codes exercise ties-to-even
+    mids = (vals[1:] + vals[:-1]) / 2
+    for v in np.concatenate([xs, mids, -mids[::7]]):
+        d = np.abs(vals - abs(v))
+        cand = np.flatnonzero(d == d.min())
+        code = cand[0] if len(cand) == 1 else [c for c in cand if c % 2 == 0][0]
+        assert O.round_lp(v, "fp8") == np.copysign(vals[code], v), v
+
+
+def test_fp8_block_scale_is_smallest_power_of_two():
+    for amax in [448.0, 449.0, 1.0, 1e-30, 3.5e5, 0.017]:
+        s = O.fp8_scale(amax)
+        assert s == 2.0 ** round(np.log2(s))
+        assert amax <= 448 * s and amax > 448 * s / 2
+    assert O.fp8_scale(0.0) == 1.0
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp16", "fp8"])
+def test_lp_blocked_factor_against_numpy_restatement(prec):
+    """One trailing update (n = 2B): the oracle's low-precision blocked
+    factorisation equals a numpy restatement of quantise-and-update."""
+    n, B = 64, 32
+    A, _, _, _ = O.generate_ref(n, seed=5)
+    got = O.gefa_blocked_lp(A, B, prec)
+    a = A.copy()
+    a[:B, :B] = O.gefa_nopvt(a[:B, :B])
+    a[:B, B:] = O.top_update(a[:B, B:], a[:B, :B])
+    a[B:, :B] = O.left_update(a[B:, :B], a[:B, :B])
+    T = a[:B, B:].T.astype(np.float64)  # vectors = columns of the top strip
+    L = a[B:, :B].astype(np.float64)    # vectors = rows of the left strip
+
+    def q(V):
+        if prec == "fp16":
+            return V.astype(np.float16).astype(np.float64)
+        if prec == "bf16":
+            return np.vectorize(_bf16_bits)(V)
+        out = np.empty_like(V)
+        for g in range(0, V.shape[0], 32):
+            blk = V[g:g + 32]
+            amax = np.max(np.abs(blk))
+            e = int(np.ceil(np.log2(amax / 448)))
+            s = 2.0 ** e
+            out[g:g + 32] = s * np.vectorize(lambda v: O.round_lp(v, "fp8"))(blk / s)
+        return out
+
+    upd = (a[B:, B:].astype(np.float64) + q(L) @ q(T).T).astype(np.float32)
+    a[B:, B:] = O.gefa_nopvt(upd)
+    assert np.array_equal(got[:B], a[:B]) and np.array_equal(got[B:, :B], a[B:, :B])
+    assert np.max(np.abs(got[B:, B:] - a[B:, B:])) <= 1e-6 * np.max(np.abs(a[B:, B:]))
+
+
+def test_lp_factor_error_ordered_by_precision():
+    """Coarser update formats move the factor further from the FP32 one."""
+    A, _, _, _ = O.generate_ref(256, seed=2)
+    ref = O.gefa_blocked(A, 64)
+    err = {p: np.max(np.abs(O.gefa_blocked_lp(A, 64, p) - ref)) for p in ("fp16", "bf16", "fp8")}
+    assert 0 < err["fp16"] < err["bf16"] < err["fp8"]
