@@ -25,6 +25,18 @@ sys.path.insert(0, ROOT)
 MFMA_FP32_PEAK_TFLOPS = 157.3  # MI355X dense FP32 matrix peak (datasheet)
 
 
+def measured_traffic(n, B):
+    """HBM bytes per trailing-update launch measured by rocprofv3 (FETCH_SIZE
+    and WRITE_SIZE passes over this same workload, scaled by the counters'
+    reading of a 4 GiB device copy); None when no measurement exists for
+    this n and block."""
+    path = os.path.join(ROOT, "profiles", f"gemm_f32_n{n}_traffic.json")
+    if B != 256 or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
+
+
 def cpu_baseline(n=5120, seed=42):
     """The reference CPU path (gefa_ref_nopvt + gesl_ref_nopvt restated in the
     oracle), one core, on a bounded sample of the same generator."""
@@ -110,6 +122,126 @@ def batched_section(ctx, count=4096, seed=7):
             "failed": st["failed"], "autotuned_threads_per_bucket": [int(x) for x in threads]}
 
 
+def serving_section(rate=None, count=600, seed=11, budget=200 * 10 ** 9):
+    """Request loop: Poisson arrivals, n log-uniform in [64, 8192], nrhs in
+    1..16, 20% of requests reusing a resident factor with new right-hand
+    sides, FP8 E4M3 trailing update with per-block scales and FP64 refinement
+    to 1e-14 n, HBM budget 200 GB.  Systems are generated on the device from
+    per-request seeds; right-hand sides and solutions cross PCIe inside the
+    measured latency.  With rate None the arrival rate is set to 70% of the
+    throughput of a short closed-loop run."""
+    import queue
+    import threading
+
+    import numpy as np
+
+    from hpcc_fpga_amd import HplError, Server
+
+    rng = np.random.default_rng(seed)
+    try:
+        srv = Server(slots=4, hbm_budget=budget, prec="fp8", max_n=8192, max_nrhs=16)
+    except HplError as e:
+        if e.status != 3:
+            raise
+        budget //= 2
+        srv = Server(slots=4, hbm_budget=budget, prec="fp8", max_n=8192, max_nrhs=16)
+
+    def draw():
+        n = int(round(np.exp(rng.uniform(np.log(64), np.log(8192)))))
+        return n, int(rng.integers(1, 17))
+
+    pool, pool_lock = [], threading.Lock()  # (factor, n) of completed kept requests
+
+    def run(arrivals):
+        """Submit at the given offsets (s); returns (records, wall seconds)."""
+        tickets = queue.Queue()
+        recs = []
+
+        def collector():
+            while True:
+                item = tickets.get()
+                if item is None:
+                    return
+                t, kind, n, nrhs = item
+                _, resp = srv.wait(t, check=False)
+                recs.append((kind, n, nrhs, resp))
+                if kind == "fresh" and resp["status"] == 0 and resp["factor"]:
+                    with pool_lock:
+                        pool.append((resp["factor"], n))
+                        if len(pool) > 32:
+                            f, _ = pool.pop(0)
+                            srv.release(f)
+
+        th = threading.Thread(target=collector)
+        th.start()
+        t0 = time.perf_counter()
+        for k, at in enumerate(arrivals):
+            dt = at - (time.perf_counter() - t0)
+            if dt > 0:
+                time.sleep(dt)
+            n, nrhs = draw()
+            reuse = None
+            if rng.random() < 0.2:
+                with pool_lock:
+                    if pool:
+                        reuse = pool[int(rng.integers(len(pool)))]
+            if reuse is not None:
+                f, n = reuse
+                b = rng.uniform(-1, 1, (n, nrhs))
+                try:
+                    t = srv.submit(b, factor=f)
+                    tickets.put((t, "reuse", n, nrhs))
+                    continue
+                except HplError:
+                    pass  # released meanwhile: serve it as a new system
+                n, nrhs = draw()
+            b = rng.uniform(-1, 1, (n, nrhs))
+            t = srv.submit(b, seed=seed * 100000 + k, keep=True)
+            tickets.put((t, "fresh", n, nrhs))
+        tickets.put(None)
+        th.join()
+        return recs, time.perf_counter() - t0
+
+    # closed-loop calibration: everything arrives at once
+    cal, cal_wall = run([0.0] * 60)
+    cap = len(cal) / cal_wall
+    lam = rate if rate else 0.7 * cap
+    arrivals = np.cumsum(rng.exponential(1.0 / lam, count))
+    print(f"[bench] serving: capacity ~{cap:.0f} req/s, Poisson rate {lam:.0f} req/s, {count} "
+          "requests", file=sys.stderr, flush=True)
+    recs, wall = run(arrivals)
+    st = srv.stats()
+    srv.close()
+    ok = [r for r in recs if r[3]["status"] == 0]
+    lat = np.array([r[3]["latency_s"] for r in ok]) * 1e3
+    flops = sum((2.0 / 3.0 * r[1] ** 3 + 2.0 * r[1] ** 2 * r[2]) if r[0] == "fresh"
+                else 2.0 * r[1] ** 2 * r[2] for r in ok)
+    out = {
+        "workload": f"{count} requests, Poisson {lam:.1f}/s, n log-uniform [64,8192], nrhs 1..16, "
+                    "20% factor reuse, FP8 E4M3 per-block-scaled update + FP64 refinement to "
+                    "1e-14 n, device-generated systems (seeded), 4 slots",
+        "hbm_budget_bytes": budget,
+        "closed_loop_capacity_req_s": round(cap, 1),
+        "arrival_rate_req_s": round(lam, 1),
+        "solves_per_s": round(len(ok) / wall, 1),
+        "effective_gflops": round(flops / wall / 1e9, 1),
+        "p50_latency_ms": round(float(np.percentile(lat, 50)), 3),
+        "p99_latency_ms": round(float(np.percentile(lat, 99)), 3),
+        "p50_queue_ms": round(float(np.percentile([r[3]["queue_s"] for r in ok], 50)) * 1e3, 3),
+        "reuse_fraction": round(sum(r[0] == "reuse" for r in recs) / max(1, len(recs)), 3),
+        "p50_latency_reuse_ms": round(float(np.percentile(
+            [r[3]["latency_s"] * 1e3 for r in ok if r[0] == "reuse"] or [0.0], 50)), 3),
+        "max_backward_error_per_n": max(r[3]["backward_error"] / r[1] for r in ok),
+        "max_iterations": max(r[3]["iterations"] for r in ok),
+        "failed": len(recs) - len(ok),
+        "max_in_flight": st["max_in_flight"],
+        "peak_factor_bytes": st["peak_factor_bytes"],
+    }
+    print(f"[bench] serving: p50 {out['p50_latency_ms']} ms p99 {out['p99_latency_ms']} ms, "
+          f"{out['solves_per_s']} solves/s", file=sys.stderr, flush=True)
+    return out
+
+
 def _hash_uniform(seed, stream, k):
     """The counter-based uniform of the generators (splitmix64 finaliser)."""
     M = (1 << 64) - 1
@@ -136,6 +268,9 @@ def main():
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the mixed-precision and batched sections")
     ap.add_argument("--mixed-n", type=int, default=65536)
+    ap.add_argument("--serve-rate", type=float, default=None,
+                    help="Poisson arrival rate (req/s); default 70%% of measured capacity")
+    ap.add_argument("--serve-count", type=int, default=600)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -189,11 +324,15 @@ def main():
     tot = np.array(t_lu) + np.array(t_sl)
     value = flops * len(tot) / float(np.sum(tot)) / 1e9
     achieved = gemm_flops / gemm_s / 1e12 if gemm_s > 0 else 0.0
+    tr = measured_traffic(n, B)
 
     extra = {}
     if not args.no_extra:
         extra["mixed_precision"] = mixed_precision_section(ctx, args.mixed_n, B)
         extra["batched"] = batched_section(ctx)
+    ctx.close()
+    if not args.no_extra:
+        extra["serving"] = serving_section(args.serve_rate, args.serve_count)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -225,14 +364,20 @@ def main():
                        "passed": bool(residn < 1 and hpl < 1)},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3),
                      "peak": MFMA_FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / MFMA_FP32_PEAK_TFLOPS, 4), "traffic": None,
-                     "kernel": "gemm_update_f32_kernel", "launches": launches,
+                     "frac": round(achieved / MFMA_FP32_PEAK_TFLOPS, 4),
+                     "traffic": round(tr["hbm_bytes_per_launch"]) if tr else None,
+                     "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
+                                     "copy-calibrated)",
+                     "algorithmic_bytes_per_launch": round(tr["algorithmic_bytes_per_launch"])
+                     if tr else None,
+                     "algorithmic_flops_per_launch": round(gemm_flops / max(1, launches)),
+                     "kernel": "gemm_f32::update_kernel<128,128,32,32,64,2>",
+                     "launches": launches, "avg_launch_us": round(gemm_s / max(1, launches) * 1e6, 1),
                      "kernel_s": round(gemm_s, 6)},
         "cpu_baseline": cpu,
         "device": dev,
     }
     line.update(extra)
-    ctx.close()
     print(json.dumps(line), flush=True)
 
 

@@ -27,11 +27,15 @@
 namespace {
 
 constexpr int NT = 256;
+constexpr int kDiagPass = 128;  // entries of the next diagonal block per gemv pass
 
+// Entries per off-diagonal workgroup: 4*ncols entries, the B rows split over
+// NT/ncols thread groups, so each thread streams B*ncols/NT rows with all of
+// its loads in flight (the sweep is bound by load latency, not bandwidth).
 template <int NR>
 struct gesl_cfg {
-    static constexpr int MAXB = NR >= 16 ? 256 : 512;       // largest block
-    static constexpr int CH = NR >= 8 ? 1024 : NR >= 4 ? 512 : 256;  // entries per off-diagonal WG
+    static constexpr int MAXB = NR >= 16 ? 256 : 512;  // largest block
+    static constexpr int CH = NR >= 16 ? 256 : NR >= 4 ? 128 : 64;
 };
 
 // sum_{p < B} A[(pbase + p) * lda + e] * v[r][p] for the entries
@@ -47,6 +51,7 @@ __device__ __forceinline__ void block_gemv(const float* __restrict__ A, int64_t 
     const int t = threadIdx.x;
     const int groups = NT / ncols;
     const int col = t % ncols, g = t / ncols;
+    if (NR == 1) nr = 1;
     f32x4 a[NR];
 #pragma unroll
     for (int r = 0; r < NR; ++r) a[r] = f32x4{0, 0, 0, 0};
@@ -54,7 +59,7 @@ __device__ __forceinline__ void block_gemv(const float* __restrict__ A, int64_t 
     const bool live = e < elim;  // elim is a multiple of 4
     if (live) {
         const float* src = A + pbase * lda + e;
-#pragma unroll 4
+#pragma unroll 16
         for (int p = g; p < B; p += groups) {
             const f32x4 m = *reinterpret_cast<const f32x4*>(src + (int64_t)p * lda);
 #pragma unroll
@@ -111,6 +116,7 @@ __device__ void diag_solve(const float* __restrict__ A, int64_t lda, int64_t Qof
     constexpr int MAXB = gesl_cfg<NR>::MAXB;
     const int t = threadIdx.x;
     const int NQ = B / 32;
+    if (NR == 1) nr = 1;
     for (int idx = t; idx < nr * B; idx += NT) {
         const int r = idx / B, e = idx % B;
         ybuf[r * MAXB + e] = X[r * ldx + Qoff + e];
@@ -126,7 +132,7 @@ __device__ void diag_solve(const float* __restrict__ A, int64_t lda, int64_t Qof
 #pragma unroll
         for (int r = 0; r < NR; ++r) a[r] = 0.0f;
         const float* col = A + Qoff * lda + Qoff + 32 * s + i;
-#pragma unroll 4
+#pragma unroll 8
         for (int p = p_lo + g; p < p_hi; p += 8) {
             const float m = col[(int64_t)p * lda];
 #pragma unroll
@@ -195,8 +201,8 @@ __global__ __launch_bounds__(NT) void gesl_fwd_kernel(const float* __restrict__ 
         const int64_t Q = (int64_t)(q + 1) * B;
         if (q >= 0) {
             // entries of block q+1: x_e += sum_p A[qoff+p][e] y_p
-            for (int e0 = 0; e0 < B; e0 += 4 * NT / 2) {
-                const int ncols = min(NT / 2, (B - e0) / 4);
+            for (int e0 = 0; e0 < B; e0 += kDiagPass) {
+                const int ncols = min(kDiagPass, B - e0) / 4;
                 block_gemv<NR, 1>(A, lda, qoff, Q + e0, ncols, B, sm.vs, MAXB, nr, sm.red, n, X,
                                   ldx);
             }
@@ -233,8 +239,8 @@ __global__ __launch_bounds__(NT) void gesl_bwd_kernel(const float* __restrict__ 
     if (blockIdx.x == 0) {
         const int64_t Q = (int64_t)(q - 1) * B;
         if (q < N) {
-            for (int e0 = 0; e0 < B; e0 += 4 * NT / 2) {
-                const int ncols = min(NT / 2, (B - e0) / 4);
+            for (int e0 = 0; e0 < B; e0 += kDiagPass) {
+                const int ncols = min(kDiagPass, B - e0) / 4;
                 block_gemv<NR, -1>(A, lda, qoff, Q + e0, ncols, B, sm.vs, MAXB, nr, sm.red, Q + B,
                                    X, ldx);
             }

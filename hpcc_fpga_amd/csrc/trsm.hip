@@ -276,6 +276,150 @@ __global__ __launch_bounds__(64) void trsm_kernel(float* __restrict__ A, int64_t
     }
 }
 
+// ----------------------------------------------------------------------------
+// The same solve with the solved blocks kept in registers (B <= 256): no LDS
+// for Y, so the kernel fits beside the trailing-update GEMM on a CU (the LDS
+// version needs B*32*4 bytes per wave).  The k order of each MFMA pair
+// follows the accumulator layout: register r of a lane holds row acc_row(r)
+// of its block, so the lane supplies that row's y as the B operand and the
+// matching coefficient as the A operand -- both z = v + C y and y = G z are
+// computed from registers.
+template <int B, typename LP>
+__global__ __launch_bounds__(64) void trsm_reg_kernel(float* __restrict__ A, int64_t lda,
+                                                      int64_t off, int64_t m,
+                                                      const float* __restrict__ DT,
+                                                      const float* __restrict__ inv,
+                                                      float* __restrict__ Lt, int64_t ldt,
+                                                      int ngroups_top, LP* __restrict__ lpT,
+                                                      LP* __restrict__ lpL, int64_t ldlp,
+                                                      float* __restrict__ scT,
+                                                      float* __restrict__ scL) {
+    constexpr int NQ = B / 32;
+    constexpr bool kFp8 = std::is_same<LP, fp8e4m3>::value;
+    constexpr bool kLp = !std::is_same<LP, float>::value && !kFp8;
+    __shared__ float Tt[32][33];  // transposes of the left strip
+    const int lane = threadIdx.x;
+    const int lr = lane & 31;
+    const bool top = (int)blockIdx.x < ngroups_top;
+    const int64_t v0 = (int64_t)(top ? blockIdx.x : blockIdx.x - ngroups_top) * 32;
+    const float* D = A + off * lda + off;
+    const float* Xc = top ? DT : D;  // coefficients read as X[p][i]
+    const int64_t ldx = top ? (int64_t)B : lda;
+    const float* G = inv + (top ? 0 : NQ * 1024);
+    float* stripT = A + off * lda + off + B;
+    float* stripL = A + (off + B) * lda + off;
+
+    f32x16 Y[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        f32x16 acc;
+        if (top) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r)
+                acc[r] = stripT[(int64_t)(q * 32 + acc_row(r, lane)) * lda + v0 + lr];
+        } else {
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const int vr = (lane >> 3) + 8 * s, c4 = (lane & 7) * 4;
+                const f32x4 v =
+                    *reinterpret_cast<const f32x4*>(stripL + (v0 + vr) * lda + q * 32 + c4);
+                Tt[vr][c4 + 0] = v[0];
+                Tt[vr][c4 + 1] = v[1];
+                Tt[vr][c4 + 2] = v[2];
+                Tt[vr][c4 + 3] = v[3];
+            }
+            wave_lds_sync();
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[r] = Tt[lr][acc_row(r, lane)];
+            wave_lds_sync();
+        }
+        // z = v + sum over solved blocks pb < q of C[q][pb] y_pb
+        float c[2][16];
+        if (q > 0) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r)
+                c[0][r] = Xc[(int64_t)acc_row(r, lane) * ldx + q * 32 + lr];
+        }
+#pragma unroll
+        for (int pb = 0; pb < q; ++pb) {
+            if (pb + 1 < q) {
+#pragma unroll
+                for (int r = 0; r < 16; ++r)
+                    c[(pb + 1) & 1][r] =
+                        Xc[(int64_t)((pb + 1) * 32 + acc_row(r, lane)) * ldx + q * 32 + lr];
+            }
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc = mfma32(c[pb & 1][r], Y[pb][r], acc);
+        }
+        // y = G_q z  (G[i][k] stored at Gq[k * 32 + i])
+        const float* Gq = G + q * 1024;
+        f32x16 y = {0};
+#pragma unroll
+        for (int r = 0; r < 16; ++r) y = mfma32(Gq[acc_row(r, lane) * 32 + lr], acc[r], y);
+        Y[q] = y;
+        if constexpr (kLp) {
+            LP* dst = (top ? lpT : lpL) + (v0 + lr) * ldlp + q * 32 + 4 * (lane >> 5);
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                struct alignas(8) P4 { LP v[4]; } p4;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) p4.v[e] = static_cast<LP>(y[4 * g + e]);
+                *reinterpret_cast<P4*>(dst + 8 * g) = p4;
+            }
+        }
+        if (top) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r)
+                stripT[(int64_t)(q * 32 + acc_row(r, lane)) * lda + v0 + lr] = y[r];
+        } else {
+            if (Lt)
+#pragma unroll
+                for (int r = 0; r < 16; ++r)
+                    Lt[(int64_t)(q * 32 + acc_row(r, lane)) * ldt + v0 + lr] = y[r];
+#pragma unroll
+            for (int r = 0; r < 16; ++r) Tt[acc_row(r, lane)][lr] = y[r];
+            wave_lds_sync();
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const int vr = (lane >> 3) + 8 * s, c4 = (lane & 7) * 4;
+                f32x4 v;
+                v[0] = Tt[c4 + 0][vr];
+                v[1] = Tt[c4 + 1][vr];
+                v[2] = Tt[c4 + 2][vr];
+                v[3] = Tt[c4 + 3][vr];
+                *reinterpret_cast<f32x4*>(stripL + (v0 + vr) * lda + q * 32 + c4) = v;
+            }
+            wave_lds_sync();
+        }
+    }
+    // FP8 copy: one power-of-two scale for the 32 vectors x B of this wave;
+    // registers 4g..4g+3 are 4 consecutive k, stored as one 32-bit word
+    if constexpr (kFp8) {
+        float am = 0.0f;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) am = fmaxf(am, fabsf(Y[q][r]));
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) am = fmaxf(am, __shfl_xor(am, o));
+        const int e = fp8_scale_exp(am);
+        const float sinv = exp2i(-e);
+        uint8_t* dst = reinterpret_cast<uint8_t*>(top ? lpT : lpL) + (v0 + lr) * ldlp +
+                       4 * (lane >> 5);
+#pragma unroll
+        for (int q = 0; q < NQ; ++q)
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                const uint32_t w = fp8_e4m3_code(Y[q][4 * g] * sinv) |
+                                   (fp8_e4m3_code(Y[q][4 * g + 1] * sinv) << 8) |
+                                   (fp8_e4m3_code(Y[q][4 * g + 2] * sinv) << 16) |
+                                   (fp8_e4m3_code(Y[q][4 * g + 3] * sinv) << 24);
+                *reinterpret_cast<uint32_t*>(dst + q * 32 + 8 * g) = w;
+            }
+        if (lane == 0) (top ? scT : scL)[v0 / 32] = exp2i(e);
+    }
+}
+
 }  // namespace
 
 hipError_t launch_prep(const float* A, int64_t lda, int64_t off, int B, float* DT, float* inv,
@@ -301,17 +445,17 @@ static hipError_t launch_trsm_t(float* A, int64_t lda, int64_t off, int64_t m, i
                                 float* scT = nullptr, float* scL = nullptr) {
     const int groups = (int)(m / 32);
     dim3 grid(2 * groups);
-#define HPL_TRSM_CASE(BB)                                                                     \
+#define HPL_TRSM_CASE(BB, KERNEL)                                                             \
     case BB:                                                                                  \
-        trsm_kernel<BB, LP><<<grid, 64, 0, s>>>(A, lda, off, m, DT, inv, Lt, ldt, groups, lpT, \
-                                                lpL, ldlp, scT, scL);                         \
+        KERNEL<BB, LP><<<grid, 64, 0, s>>>(A, lda, off, m, DT, inv, Lt, ldt, groups, lpT, lpL,  \
+                                           ldlp, scT, scL);                                   \
         break;
     switch (B) {
-        HPL_TRSM_CASE(32)
-        HPL_TRSM_CASE(64)
-        HPL_TRSM_CASE(128)
-        HPL_TRSM_CASE(256)
-        HPL_TRSM_CASE(512)
+        HPL_TRSM_CASE(32, trsm_reg_kernel)
+        HPL_TRSM_CASE(64, trsm_reg_kernel)
+        HPL_TRSM_CASE(128, trsm_reg_kernel)
+        HPL_TRSM_CASE(256, trsm_reg_kernel)
+        HPL_TRSM_CASE(512, trsm_kernel)
         default: return hipErrorInvalidValue;
     }
 #undef HPL_TRSM_CASE

@@ -90,7 +90,9 @@ def test_resident_factor_reuse_and_release(server):
 
 def test_budget_refusal():
     _say("serve budget")
-    with Server(slots=1, hbm_budget=int(0.3 * GB), prec="fp8", max_n=4096, max_nrhs=1) as s:
+    # 0.1 GB: the slot workspace (~10 MB at n <= 4096) plus a 90 MB factor
+    # arena -- an order-4096 factor (~136 MB) can never fit, an order-512 one can
+    with Server(slots=1, hbm_budget=int(0.1 * GB), prec="fp8", max_n=4096, max_nrhs=1) as s:
         with pytest.raises(HplError) as e:
             s.submit(np.ones(4096), seed=1)
         assert e.value.status == 5  # HPL_ERR_BUDGET
