@@ -214,17 +214,27 @@ hpl_status factor_diag(hpl_ctx* c, hipStream_t st, int64_t off, uint32_t B, floa
     return factor_diag(c, st, off + h, h, next);
 }
 
-// Panel of step k: factor the diagonal block, prepare its inverses, solve the
-// top and left strips (the left one also, transposed, into its Lt slot with
-// absolute row indices).
-hpl_status panel(hpl_ctx* c, hipStream_t st, int64_t k, uint32_t B) {
-    const int64_t n = (int64_t)c->n, lda = (int64_t)c->lda;
+// Panel of step k, part 1: factor the diagonal block and prepare its
+// inverses (latency-bound, a few workgroups).
+hpl_status panel_diag(hpl_ctx* c, hipStream_t st, int64_t k, uint32_t B) {
+    const int64_t lda = (int64_t)c->lda;
     const int64_t off = k * B;
     hpl_status r = factor_diag(c, st, off, B, c->dInner);
     if (r) return r;
     float* inv = c->dInv + k * (int64_t)(3 * (B / 32) * 1024);
     float* DT = c->dDT + (k & 1) * (int64_t)B * B;
     HIPCHK(c, launch_prep(c->dA, lda, off, B, DT, inv, st));
+    return HPL_SUCCESS;
+}
+
+// Panel of step k, part 2: solve the top and left strips (the left one also,
+// transposed, into its Lt slot with absolute row indices) -- one wave per 32
+// vectors, as wide as the trailing matrix.
+hpl_status panel_strips(hpl_ctx* c, hipStream_t st, int64_t k, uint32_t B) {
+    const int64_t n = (int64_t)c->n, lda = (int64_t)c->lda;
+    const int64_t off = k * B;
+    float* inv = c->dInv + k * (int64_t)(3 * (B / 32) * 1024);
+    float* DT = c->dDT + (k & 1) * (int64_t)B * B;
     const int64_t m = n - off - B;
     if (m > 0) {
         const int64_t slot = lt_slot(c, k);
@@ -249,13 +259,21 @@ hpl_status panel(hpl_ctx* c, hipStream_t st, int64_t k, uint32_t B) {
     return HPL_SUCCESS;
 }
 
+hpl_status panel(hpl_ctx* c, hipStream_t st, int64_t k, uint32_t B) {
+    hpl_status r = panel_diag(c, st, k, B);
+    return r ? r : panel_strips(c, st, k, B);
+}
+
 // Blocked right-looking schedule with depth-1 look-ahead (the pattern of the
 // reference's XRT backend, execution_xrt_pcie.hpp:328-415) and a delayed bulk
 // update.  At step k the row and column blocks of the next panel receive all
-// pending updates; the panel of step k+1 then runs on stream s1 while stream
-// s0 updates the rest of the trailing matrix -- but only every `depth` steps,
-// applying `depth` steps' updates in one GEMM with K = depth * B.  Every
-// element still receives each update exactly once, in step order.
+// pending updates; the panel of step k+1 (diagonal block and strip solves)
+// then runs on stream s1 while stream s0 updates the rest of the trailing
+// matrix -- but only every `depth` steps, applying `depth` steps' updates in
+// one GEMM with K = depth * B.  Every element still receives each update
+// exactly once, in step order.  (Running the strip solves on s0 after the
+// bulk update instead, with the whole device, measured 1% slower at
+// n = 32768: they then sit on the critical path.)
 hpl_status gefa_schedule(hpl_ctx* c, uint32_t B) {
     const int64_t n = (int64_t)c->n;
     const int64_t N = n / B;
