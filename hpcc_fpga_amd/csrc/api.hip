@@ -147,12 +147,35 @@ int64_t lt_slot(const hpl_ctx* c, int64_t k) {
 // applying the pending steps k0 .. k0 + K/B - 1 (their operands are stacked
 // contiguously by the slot layout).  FP32 operands: Lt slots (k-major) and the
 // top strips in A; 16-bit operands: the k-contiguous copies.
+// An update region: rows [r0, r0+M), columns [c0, c0+N) of the trailing matrix.
+struct UpdRegion {
+    int64_t r0, c0;
+    int M, N;
+};
+
+// Trailing update of one region, or of two in a single launch (b.M > 0).
+hpl_status launch_update2(hpl_ctx* c, hipStream_t st, int64_t k0, UpdRegion a, UpdRegion b, int K,
+                          uint32_t B);
+
 hpl_status launch_update(hpl_ctx* c, hipStream_t st, int64_t k0, int64_t r0, int64_t c0, int M,
                          int N, int K, uint32_t B) {
-    if (M <= 0 || N <= 0) return HPL_SUCCESS;
+    return launch_update2(c, st, k0, UpdRegion{r0, c0, M, N}, UpdRegion{0, 0, 0, 0}, K, B);
+}
+
+hpl_status launch_update2(hpl_ctx* c, hipStream_t st, int64_t k0, UpdRegion a, UpdRegion b, int K,
+                          uint32_t B) {
+    if (a.M <= 0 || a.N <= 0) {
+        a = b;
+        b = UpdRegion{0, 0, 0, 0};
+    }
+    if (a.M <= 0 || a.N <= 0) return HPL_SUCCESS;
+    if (b.M <= 0 || b.N <= 0) b = UpdRegion{0, 0, 0, 0};
     const int64_t n = (int64_t)c->n, lda = (int64_t)c->lda;
+    const int64_t r0 = a.r0, c0 = a.c0;
+    const int M = a.M, N = a.N;
     float* C = c->dA + r0 * lda + c0;
-    hipEvent_t a = nullptr, b = nullptr;
+    float* C2 = c->dA + b.r0 * lda + b.c0;
+    hipEvent_t ev_beg = nullptr, ev_end = nullptr;
     if (c->prof) {
         if (c->ev_used == c->ev.size()) {
             hipEvent_t x, y;
@@ -160,34 +183,39 @@ hpl_status launch_update(hpl_ctx* c, hipStream_t st, int64_t k0, int64_t r0, int
             HIPCHK(c, hipEventCreate(&y));
             c->ev.emplace_back(x, y);
         }
-        a = c->ev[c->ev_used].first;
-        b = c->ev[c->ev_used].second;
+        ev_beg = c->ev[c->ev_used].first;
+        ev_end = c->ev[c->ev_used].second;
         c->ev_used++;
-        HIPCHK(c, hipEventRecord(a, st));
+        HIPCHK(c, hipEventRecord(ev_beg, st));
     }
     const int64_t slot = lt_slot(c, k0);
     if (c->prec == 0) {
-        const float* Lt = c->dLt + slot * (int64_t)B * n + r0;
-        const float* T = c->dA + k0 * B * lda + c0;
-        HIPCHK(c, launch_gemm_update_f32(C, lda, Lt, n, T, lda, M, N, K, st));
+        const float* Lt = c->dLt + slot * (int64_t)B * n;
+        const float* T = c->dA + k0 * B * lda;
+        HIPCHK(c, launch_gemm_update_f32_2(C, lda, Lt + r0, n, T + c0, lda, M, N, C2, Lt + b.r0,
+                                           T + b.c0, b.M, b.N, K, st));
     } else {
         const size_t esz = lp_elem_bytes(c->prec);
-        const char* L = static_cast<const char*>(c->dLpL) + (slot * n * B + r0 * B) * esz;
-        const char* T = static_cast<const char*>(c->dLpT) + (slot * n * B + c0 * B) * esz;
+        const char* Lb = static_cast<const char*>(c->dLpL) + slot * n * B * esz;
+        const char* Tb = static_cast<const char*>(c->dLpT) + slot * n * B * esz;
+        const int64_t ns = n / 32;
+        const float* sL = c->dScL ? c->dScL + slot * ns : nullptr;
+        const float* sT = c->dScT ? c->dScT + slot * ns : nullptr;
+        LpRegion2 r2{C2, Lb + b.r0 * B * esz, Tb + b.c0 * B * esz,
+                     sL ? sL + b.r0 / 32 : nullptr, sT ? sT + b.c0 / 32 : nullptr, b.M, b.N};
         if (c->prec == HPL_PREC_FP8) {
-            const int64_t ns = n / 32;
-            HIPCHK(c, launch_gemm_update_fp8(C, lda, L, B, n * B, T, B, n * B,
-                                             c->dScL + slot * ns + r0 / 32,
-                                             c->dScT + slot * ns + c0 / 32, ns, M, N, K, (int)B,
-                                             st));
+            HIPCHK(c, launch_gemm_update_fp8(C, lda, Lb + r0 * B * esz, B, n * B,
+                                             Tb + c0 * B * esz, B, n * B, sL + r0 / 32,
+                                             sT + c0 / 32, ns, M, N, K, (int)B, st, &r2));
         } else {
-            HIPCHK(c, launch_gemm_update_lp(c->prec, C, lda, L, B, n * B, T, B, n * B, M, N, K,
-                                            (int)B, st));
+            HIPCHK(c, launch_gemm_update_lp(c->prec, C, lda, Lb + r0 * B * esz, B, n * B,
+                                            Tb + c0 * B * esz, B, n * B, M, N, K, (int)B, st,
+                                            &r2));
         }
     }
     if (c->prof) {
-        HIPCHK(c, hipEventRecord(b, st));
-        c->prof_flops += 2.0 * M * (double)N * K;
+        HIPCHK(c, hipEventRecord(ev_end, st));
+        c->prof_flops += 2.0 * ((double)M * N + (double)b.M * b.N) * K;
     }
     return HPL_SUCCESS;
 }
@@ -293,10 +321,9 @@ hpl_status gefa_schedule(hpl_ctx* c, uint32_t B) {
             if (r) return r;
             continue;
         }
-        // next panel's region: row block k+1 and column block k+1
-        r = launch_update(c, c->s0, k0, k1, k1, (int)B, (int)m, K, B);
-        if (r) return r;
-        r = launch_update(c, c->s0, k0, k1 + B, k1, (int)(m - B), (int)B, K, B);
+        // next panel's region: row block k+1 and column block k+1, one launch
+        r = launch_update2(c, c->s0, k0, UpdRegion{k1, k1, (int)B, (int)m},
+                           UpdRegion{k1 + B, k1, (int)(m - B), (int)B}, K, B);
         if (r) return r;
         HIPCHK(c, hipEventRecord(c->evA, c->s0));
         HIPCHK(c, hipStreamWaitEvent(c->s1, c->evA, 0));

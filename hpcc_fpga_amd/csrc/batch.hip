@@ -196,35 +196,54 @@ __global__ __launch_bounds__(NT) void batch_solve_kernel(
                     for (int i = 0; i < 32; ++i) Left[(size_t)r * LDT + i] = (T)x[i];
                 }
                 __syncthreads();
-                // trailing update on MFMA, 32 x 32 tiles over the waves
-                const int tiles = m / 32;
-                for (int tt = wave; tt < tiles * tiles; tt += NT / 64) {
-                    const int ti = tt / tiles, tj = tt % tiles;
-                    float* cbase = a + (int64_t)(r0 + ti * 32) * npad + r0 + tj * 32;
-                    f32x16 acc;
+                // trailing update on MFMA, 32 x 32 tiles over the waves; a wave
+                // takes TPW tiles at a time so their loads are in flight together
+                // (the update streams the trailing matrix through L2 and is
+                // bound by load latency, not by the MFMAs)
+                constexpr int TPW = 4;
+                const int tiles = m / 32, ntile = tiles * tiles;
+                for (int tt0 = wave * TPW; tt0 < ntile; tt0 += (NT / 64) * TPW) {
+                    f32x16 acc[TPW];
+                    float* cbase[TPW];
 #pragma unroll
-                    for (int rr = 0; rr < 16; ++rr)
-                        acc[rr] = cbase[(int64_t)acc_row(rr, lane) * npad + lr];
-                    if constexpr (F16) {
+                    for (int u = 0; u < TPW; ++u) {
+                        const int tt = min(tt0 + u, ntile - 1);  // clamped: in bounds, unused
+                        cbase[u] = a + (int64_t)(r0 + (tt / tiles) * 32) * npad + r0 +
+                                   (tt % tiles) * 32;
 #pragma unroll
-                        for (int ks = 0; ks < 2; ++ks) {
-                            const f16x8 av = *reinterpret_cast<const f16x8*>(
-                                &Left[(size_t)(ti * 32 + lr) * LDT + ks * 16 + 8 * lh]);
-                            const f16x8 bv = *reinterpret_cast<const f16x8*>(
-                                &TopT[(size_t)(tj * 32 + lr) * LDT + ks * 16 + 8 * lh]);
-                            acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(av, bv, acc, 0, 0, 0);
-                        }
-                    } else {
+                        for (int rr = 0; rr < 16; ++rr)
+                            acc[u][rr] = cbase[u][(int64_t)acc_row(rr, lane) * npad + lr];
+                    }
 #pragma unroll
-                        for (int kp = 0; kp < 16; ++kp) {
-                            const float av = Left[(size_t)(ti * 32 + lr) * LDT + 2 * kp + lh];
-                            const float bv = TopT[(size_t)(tj * 32 + lr) * LDT + 2 * kp + lh];
-                            acc = mfma32(av, bv, acc);
+                    for (int u = 0; u < TPW; ++u) {
+                        const int tt = min(tt0 + u, ntile - 1);
+                        const int ti = tt / tiles, tj = tt % tiles;
+                        if constexpr (F16) {
+#pragma unroll
+                            for (int ks = 0; ks < 2; ++ks) {
+                                const f16x8 av = *reinterpret_cast<const f16x8*>(
+                                    &Left[(size_t)(ti * 32 + lr) * LDT + ks * 16 + 8 * lh]);
+                                const f16x8 bv = *reinterpret_cast<const f16x8*>(
+                                    &TopT[(size_t)(tj * 32 + lr) * LDT + ks * 16 + 8 * lh]);
+                                acc[u] = __builtin_amdgcn_mfma_f32_32x32x16_f16(av, bv, acc[u], 0,
+                                                                               0, 0);
+                            }
+                        } else {
+#pragma unroll
+                            for (int kp = 0; kp < 16; ++kp) {
+                                const float av = Left[(size_t)(ti * 32 + lr) * LDT + 2 * kp + lh];
+                                const float bv = TopT[(size_t)(tj * 32 + lr) * LDT + 2 * kp + lh];
+                                acc[u] = mfma32(av, bv, acc[u]);
+                            }
                         }
                     }
 #pragma unroll
-                    for (int rr = 0; rr < 16; ++rr)
-                        cbase[(int64_t)acc_row(rr, lane) * npad + lr] = acc[rr];
+                    for (int u = 0; u < TPW; ++u) {
+                        if (tt0 + u >= ntile) break;  // wave-uniform
+#pragma unroll
+                        for (int rr = 0; rr < 16; ++rr)
+                            cbase[u][(int64_t)acc_row(rr, lane) * npad + lr] = acc[u][rr];
+                    }
                 }
             }
             __syncthreads();

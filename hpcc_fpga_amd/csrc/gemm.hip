@@ -18,3 +18,11 @@ hipError_t launch_gemm_update_f32(float* C, int64_t ldc, const float* Lt, int64_
                                   hipStream_t s) {
     return gemm_f32::launch<128, 128, 32, 32, 64, 2, false>(C, ldc, Lt, ldl, T, ldt, M, N, K, s);
 }
+
+hipError_t launch_gemm_update_f32_2(float* C, int64_t ldc, const float* Lt, int64_t ldl,
+                                    const float* T, int64_t ldt, int M, int N, float* C2,
+                                    const float* Lt2, const float* T2, int M2, int N2, int K,
+                                    hipStream_t s) {
+    return gemm_f32::launch2<128, 128, 32, 32, 64, 2, false>(
+        C, ldc, Lt, ldl, T, ldt, M, N, K, gemm_f32::Region{C2, Lt2, T2, M2, N2}, s);
+}

@@ -37,10 +37,20 @@ __device__ __forceinline__ void tile_coords(int bid, int nblk, int tiles_m, int 
     tn = in_g / gsize;
 }
 
+// A second, independent region of the same shape class (same leading
+// dimensions and K) served by the same launch: workgroups past the first
+// region's tiles take the second region's.  Empty when M = 0.
+struct Region {
+    float* C;
+    const float* At;
+    const float* Bt;
+    int M, N;
+};
+
 template <int BM, int BN, int BK, int WM, int WN, int MINB, bool CPRE, bool TRANS = false>
 __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64, MINB) void update_kernel(
     float* __restrict__ C, int64_t ldc, const float* __restrict__ At, int64_t lda,
-    const float* __restrict__ Bt, int64_t ldb, int M, int N, int K) {
+    const float* __restrict__ Bt, int64_t ldb, int M, int N, int K, Region r2) {
     constexpr int WAVES_M = BM / WM, WAVES_N = BN / WN;
     constexpr int NT = WAVES_M * WAVES_N * 64;
     constexpr int FM = WM / 32, FN = WN / 32;  // MFMA blocks per wave
@@ -51,9 +61,21 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64, MINB) void update_kerne
     __shared__ float As[2][BK][BM];
     __shared__ float Bs[2][BK][BN];
 
-    const int tiles_m = (M + BM - 1) / BM, tiles_n = (N + BN - 1) / BN;
+    int tiles_m = (M + BM - 1) / BM, tiles_n = (N + BN - 1) / BN;
+    int bid = blockIdx.x, nblk = tiles_m * tiles_n;
+    if (bid >= nblk) {  // second region
+        bid -= nblk;
+        nblk = gridDim.x - nblk;
+        C = r2.C;
+        At = r2.At;
+        Bt = r2.Bt;
+        M = r2.M;
+        N = r2.N;
+        tiles_m = (M + BM - 1) / BM;
+        tiles_n = (N + BN - 1) / BN;
+    }
     int tm, tn;
-    tile_coords(blockIdx.x, gridDim.x, tiles_m, tiles_n, tm, tn);
+    tile_coords(bid, nblk, tiles_m, tiles_n, tm, tn);
     const int m0 = tm * BM, n0 = tn * BN;
 
     const int tid = threadIdx.x;
@@ -204,15 +226,35 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64, MINB) void update_kerne
             }
 }
 
+// One launch over region (C, At, Bt, M, N) and the optional second region r2.
+template <int BM, int BN, int BK, int WM, int WN, int MINB, bool CPRE, bool TRANS = false>
+inline hipError_t launch2(float* C, int64_t ldc, const float* At, int64_t lda, const float* Bt,
+                          int64_t ldb, int M, int N, int K, Region r2, hipStream_t s) {
+    if (r2.M <= 0 || r2.N <= 0) r2.M = r2.N = 0;
+    if (M <= 0 || N <= 0) {  // only the second region (or nothing)
+        if (r2.M == 0) return hipSuccess;
+        C = r2.C;
+        At = r2.At;
+        Bt = r2.Bt;
+        M = r2.M;
+        N = r2.N;
+        r2.M = r2.N = 0;
+    }
+    if (K % BK != 0 || (M % 4) || (N % 4) || (r2.M % 4) || (r2.N % 4)) return hipErrorInvalidValue;
+    constexpr int NT = (BM / WM) * (BN / WN) * 64;
+    const int tiles = ((M + BM - 1) / BM) * ((N + BN - 1) / BN) +
+                      ((r2.M + BM - 1) / BM) * ((r2.N + BN - 1) / BN);
+    update_kernel<BM, BN, BK, WM, WN, MINB, CPRE, TRANS>
+        <<<tiles, NT, 0, s>>>(C, ldc, At, lda, Bt, ldb, M, N, K, r2);
+    return hipGetLastError();
+}
+
 template <int BM, int BN, int BK, int WM, int WN, int MINB, bool CPRE, bool TRANS = false>
 inline hipError_t launch(float* C, int64_t ldc, const float* At, int64_t lda, const float* Bt,
                          int64_t ldb, int M, int N, int K, hipStream_t s) {
-    if (M <= 0 || N <= 0) return hipSuccess;
-    if (K % BK != 0 || (M % 4) || (N % 4)) return hipErrorInvalidValue;
-    constexpr int NT = (BM / WM) * (BN / WN) * 64;
-    const int tiles = ((M + BM - 1) / BM) * ((N + BN - 1) / BN);
-    update_kernel<BM, BN, BK, WM, WN, MINB, CPRE, TRANS><<<tiles, NT, 0, s>>>(C, ldc, At, lda, Bt, ldb, M, N, K);
-    return hipGetLastError();
+    return launch2<BM, BN, BK, WM, WN, MINB, CPRE, TRANS>(C, ldc, At, lda, Bt, ldb, M, N, K,
+                                                          Region{nullptr, nullptr, nullptr, 0, 0},
+                                                          s);
 }
 
 }  // namespace gemm_f32

@@ -42,6 +42,69 @@ struct lp_traits<_Float16> {
 };
 
 constexpr int BM = 128, BN = 128, BK = 64, NT = 256, WM = 64, WN = 64;
+
+// Optional second region served by the same launch (workgroups past the
+// first region's tiles); empty when M = 0.
+struct LpRegion {
+    float* C;
+    const void* L;
+    const void* T;
+    const float* scL;
+    const float* scT;
+    int M, N;
+};
+
+// The region of this workgroup, with its tile index and tile count.
+template <typename TE>
+struct RegionSel {
+    float* C;
+    const TE* L;
+    const TE* T;
+    const float* scL;
+    const float* scT;
+    int M, N, bid, nblk;
+};
+template <typename TE>
+__device__ __forceinline__ RegionSel<TE> pick_region(float* C, const TE* L, const TE* Tt,
+                                                     const float* scL, const float* scT, int M,
+                                                     int N, const LpRegion& r2) {
+    RegionSel<TE> r{C, L, Tt, scL, scT, M, N, (int)blockIdx.x,
+                    ((M + BM - 1) / BM) * ((N + BN - 1) / BN)};
+    if (r.bid >= r.nblk) {
+        r.bid -= r.nblk;
+        r.nblk = gridDim.x - r.nblk;
+        r.C = r2.C;
+        r.L = static_cast<const TE*>(r2.L);
+        r.T = static_cast<const TE*>(r2.T);
+        r.scL = r2.scL;
+        r.scT = r2.scT;
+        r.M = r2.M;
+        r.N = r2.N;
+    }
+    return r;
+}
+
+// Normalise the pair of regions (first non-empty one first); false when
+// both are empty.
+bool order_regions(float*& C, const void*& L, const void*& Tt, const float*& scL,
+                   const float*& scT, int& M, int& N, LpRegion& r2) {
+    if (r2.M <= 0 || r2.N <= 0) r2.M = r2.N = 0;
+    if (M <= 0 || N <= 0) {
+        if (r2.M == 0) return false;
+        C = r2.C;
+        L = r2.L;
+        Tt = r2.T;
+        scL = r2.scL;
+        scT = r2.scT;
+        M = r2.M;
+        N = r2.N;
+        r2.M = r2.N = 0;
+    }
+    return true;
+}
+
+int region_tiles(int M, int N) { return ((M + BM - 1) / BM) * ((N + BN - 1) / BN); }
+
 constexpr int LDR = BK + 8;  // padded LDS row (elements)
 
 __device__ __forceinline__ void tile_coords(int bid, int nblk, int tiles_m, int tiles_n, int& tm,
@@ -61,14 +124,22 @@ __device__ __forceinline__ void tile_coords(int bid, int nblk, int tiles_m, int 
 template <typename T>
 __global__ __launch_bounds__(NT, 2) void update_lp_kernel(
     float* __restrict__ C, int64_t ldc, const T* __restrict__ L, int64_t l_ld, int64_t l_slot,
-    const T* __restrict__ Tt, int64_t t_ld, int64_t t_slot, int M, int N, int K, int Bblk) {
+    const T* __restrict__ Tt, int64_t t_ld, int64_t t_slot, int M, int N, int K, int Bblk,
+    LpRegion r2) {
     typedef typename lp_traits<T>::vec vec;
     __shared__ __attribute__((aligned(16))) T As[2][BM * LDR];
     __shared__ __attribute__((aligned(16))) T Bs[2][BN * LDR];
 
+    const auto rs = pick_region<T>(C, L, Tt, nullptr, nullptr, M, N, r2);
+    C = rs.C;
+    L = rs.L;
+    Tt = rs.T;
+    M = rs.M;
+    N = rs.N;
+    const int bid = rs.bid, nblk = rs.nblk;
     const int tiles_m = (M + BM - 1) / BM, tiles_n = (N + BN - 1) / BN;
     int tm, tn;
-    tile_coords(blockIdx.x, gridDim.x, tiles_m, tiles_n, tm, tn);
+    tile_coords(bid, nblk, tiles_m, tiles_n, tm, tn);
     const int m0 = tm * BM, n0 = tn * BN;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
@@ -157,12 +228,15 @@ __global__ __launch_bounds__(NT, 2) void update_lp_kernel(
 template <typename T>
 hipError_t launch_t(float* C, int64_t ldc, const void* L, int64_t l_ld, int64_t l_slot,
                     const void* Tt, int64_t t_ld, int64_t t_slot, int M, int N, int K, int Bblk,
-                    hipStream_t s) {
-    if (M <= 0 || N <= 0) return hipSuccess;
+                    hipStream_t s, LpRegion r2) {
+    const float* scL = nullptr;
+    const float* scT = nullptr;
+    if (!order_regions(C, L, Tt, scL, scT, M, N, r2)) return hipSuccess;
     if (K % BK || Bblk % BK || (l_ld % 8) || (t_ld % 8)) return hipErrorInvalidValue;
-    const int tiles = ((M + BM - 1) / BM) * ((N + BN - 1) / BN);
+    const int tiles = region_tiles(M, N) + region_tiles(r2.M, r2.N);
     update_lp_kernel<T><<<tiles, NT, 0, s>>>(C, ldc, static_cast<const T*>(L), l_ld, l_slot,
-                                            static_cast<const T*>(Tt), t_ld, t_slot, M, N, K, Bblk);
+                                            static_cast<const T*>(Tt), t_ld, t_slot, M, N, K, Bblk,
+                                            r2);
     return hipGetLastError();
 }
 
@@ -182,13 +256,22 @@ __global__ __launch_bounds__(NT, 2) void update_fp8_kernel(
     float* __restrict__ C, int64_t ldc, const uint8_t* __restrict__ L, int64_t l_ld,
     int64_t l_slot, const uint8_t* __restrict__ Tt, int64_t t_ld, int64_t t_slot,
     const float* __restrict__ scL, const float* __restrict__ scT, int64_t sc_slot, int M, int N,
-    int K, int Bblk) {
+    int K, int Bblk, LpRegion r2) {
     __shared__ __attribute__((aligned(16))) uint8_t As[2][BM * LDR8];
     __shared__ __attribute__((aligned(16))) uint8_t Bs[2][BN * LDR8];
 
+    const auto rs = pick_region<uint8_t>(C, L, Tt, scL, scT, M, N, r2);
+    C = rs.C;
+    L = rs.L;
+    Tt = rs.T;
+    scL = rs.scL;
+    scT = rs.scT;
+    M = rs.M;
+    N = rs.N;
+    const int bid = rs.bid, nblk = rs.nblk;
     const int tiles_m = (M + BM - 1) / BM, tiles_n = (N + BN - 1) / BN;
     int tm, tn;
-    tile_coords(blockIdx.x, gridDim.x, tiles_m, tiles_n, tm, tn);
+    tile_coords(bid, nblk, tiles_m, tiles_n, tm, tn);
     const int m0 = tm * BM, n0 = tn * BN;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
@@ -301,23 +384,29 @@ __global__ __launch_bounds__(NT, 2) void update_fp8_kernel(
 hipError_t launch_gemm_update_fp8(float* C, int64_t ldc, const void* L, int64_t l_ld,
                                   int64_t l_slot, const void* Tt, int64_t t_ld, int64_t t_slot,
                                   const float* scL, const float* scT, int64_t sc_slot, int M,
-                                  int N, int K, int Bblk, hipStream_t s) {
-    if (M <= 0 || N <= 0) return hipSuccess;
-    if (K % Bblk || Bblk % BK8 || (l_ld % 16) || (t_ld % 16) || (M % 32) || (N % 32))
+                                  int N, int K, int Bblk, hipStream_t s, const LpRegion2* r2in) {
+    LpRegion r2{nullptr, nullptr, nullptr, nullptr, nullptr, 0, 0};
+    if (r2in) r2 = LpRegion{r2in->C, r2in->L, r2in->T, r2in->scL, r2in->scT, r2in->M, r2in->N};
+    if (!order_regions(C, L, Tt, scL, scT, M, N, r2)) return hipSuccess;
+    if (K % Bblk || Bblk % BK8 || (l_ld % 16) || (t_ld % 16) || (M % 32) || (N % 32) ||
+        (r2.M % 32) || (r2.N % 32))
         return hipErrorInvalidValue;
-    const int tiles = ((M + BM - 1) / BM) * ((N + BN - 1) / BN);
+    const int tiles = region_tiles(M, N) + region_tiles(r2.M, r2.N);
     update_fp8_kernel<<<tiles, NT, 0, s>>>(C, ldc, static_cast<const uint8_t*>(L), l_ld, l_slot,
                                            static_cast<const uint8_t*>(Tt), t_ld, t_slot, scL, scT,
-                                           sc_slot, M, N, K, Bblk);
+                                           sc_slot, M, N, K, Bblk, r2);
     return hipGetLastError();
 }
 
 hipError_t launch_gemm_update_lp(int prec, float* C, int64_t ldc, const void* L, int64_t l_ld,
                                  int64_t l_slot, const void* Tt, int64_t t_ld, int64_t t_slot,
-                                 int M, int N, int K, int Bblk, hipStream_t s) {
+                                 int M, int N, int K, int Bblk, hipStream_t s,
+                                 const LpRegion2* r2in) {
+    LpRegion r2{nullptr, nullptr, nullptr, nullptr, nullptr, 0, 0};
+    if (r2in) r2 = LpRegion{r2in->C, r2in->L, r2in->T, nullptr, nullptr, r2in->M, r2in->N};
     if (prec == 1)
-        return launch_t<__bf16>(C, ldc, L, l_ld, l_slot, Tt, t_ld, t_slot, M, N, K, Bblk, s);
+        return launch_t<__bf16>(C, ldc, L, l_ld, l_slot, Tt, t_ld, t_slot, M, N, K, Bblk, s, r2);
     if (prec == 2)
-        return launch_t<_Float16>(C, ldc, L, l_ld, l_slot, Tt, t_ld, t_slot, M, N, K, Bblk, s);
+        return launch_t<_Float16>(C, ldc, L, l_ld, l_slot, Tt, t_ld, t_slot, M, N, K, Bblk, s, r2);
     return hipErrorInvalidValue;
 }

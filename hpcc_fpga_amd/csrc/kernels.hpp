@@ -25,11 +25,28 @@ hipError_t launch_trsm(float* A, int64_t lda, int64_t off, int64_t m, int B, con
 hipError_t launch_gemm_update_f32(float* C, int64_t ldc, const float* Lt, int64_t ldl,
                                   const float* T, int64_t ldt, int M, int N, int K,
                                   hipStream_t s);
+// the same over two regions in one launch (same leading dimensions and K)
+hipError_t launch_gemm_update_f32_2(float* C, int64_t ldc, const float* Lt, int64_t ldl,
+                                    const float* T, int64_t ldt, int M, int N, float* C2,
+                                    const float* Lt2, const float* T2, int M2, int N2, int K,
+                                    hipStream_t s);
+
+// Second region of a low-precision trailing-update launch (same leading
+// dimensions, slots and K as the first; FP8 scales offset like L and T).
+struct LpRegion2 {
+    float* C;
+    const void* L;
+    const void* T;
+    const float* scL;
+    const float* scT;
+    int M, N;
+};
 
 // mixed-precision trailing update (gemm_lp.hip): prec 1 = BF16, 2 = FP16
 hipError_t launch_gemm_update_lp(int prec, float* C, int64_t ldc, const void* L, int64_t l_ld,
                                  int64_t l_slot, const void* Tt, int64_t t_ld, int64_t t_slot,
-                                 int M, int N, int K, int Bblk, hipStream_t s);
+                                 int M, int N, int K, int Bblk, hipStream_t s,
+                                 const LpRegion2* r2 = nullptr);
 
 // FP8 trailing update (gemm_lp.hip): C[r][c] += sum over the K / Bblk steps p
 // of scL[p][r/32] * scT[p][c/32] * sum_k L[r][k] T[c][k]  (E4M3 codes, FP32
@@ -37,7 +54,8 @@ hipError_t launch_gemm_update_lp(int prec, float* C, int64_t ldc, const void* L,
 hipError_t launch_gemm_update_fp8(float* C, int64_t ldc, const void* L, int64_t l_ld,
                                   int64_t l_slot, const void* Tt, int64_t t_ld, int64_t t_slot,
                                   const float* scL, const float* scT, int64_t sc_slot, int M,
-                                  int N, int K, int Bblk, hipStream_t s);
+                                  int N, int K, int Bblk, hipStream_t s,
+                                  const LpRegion2* r2 = nullptr);
 
 // substitution (gesl.hip)
 hipError_t launch_gesl(const float* A, int64_t lda, int64_t n, int B, const float* inv,
