@@ -371,7 +371,7 @@ def main():
                      "algorithmic_bytes_per_launch": round(tr["algorithmic_bytes_per_launch"])
                      if tr else None,
                      "algorithmic_flops_per_launch": round(gemm_flops / max(1, launches)),
-                     "kernel": "gemm_f32::update_kernel<128,128,32,32,64,2>",
+                     "kernel": "gemm_f32::update_kernel<128,128,16,32,64,2>",
                      "launches": launches, "avg_launch_us": round(gemm_s / max(1, launches) * 1e6, 1),
                      "kernel_s": round(gemm_s, 6)},
         "cpu_baseline": cpu,
