@@ -49,9 +49,11 @@ __device__ __forceinline__ double block_max(double v, double* red) {
     return r;
 }
 
-// Solve M x = rhs in place on x (FP32), factor in A (npad, dgefa encoding).
+// Solve M x = rhs in place on x (FP32, in LDS), factor in A (npad, dgefa
+// encoding).  The GEMV loops keep all 32 factor loads of a block in flight.
 template <int NT>
-__device__ void substitute(const float* __restrict__ A, int npad, float* __restrict__ x) {
+__device__ __forceinline__ void substitute(const float* __restrict__ A, int npad,
+                                           float* __restrict__ x) {
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     // forward: x_e += sum_p A[p][e] x_p, block by block
     for (int k0 = 0; k0 < npad; k0 += 32) {
@@ -66,7 +68,7 @@ __device__ void substitute(const float* __restrict__ A, int npad, float* __restr
         __syncthreads();
         for (int e = k0 + 32 + t; e < npad; e += NT) {
             float a = 0.0f;
-#pragma unroll 8
+#pragma unroll
             for (int p = 0; p < 32; ++p) a = fmaf(A[(int64_t)(k0 + p) * npad + e], x[k0 + p], a);
             x[e] += a;
         }
@@ -86,7 +88,7 @@ __device__ void substitute(const float* __restrict__ A, int npad, float* __restr
         __syncthreads();
         for (int e = t; e < k0; e += NT) {
             float a = 0.0f;
-#pragma unroll 8
+#pragma unroll
             for (int p = 0; p < 32; ++p) a = fmaf(A[(int64_t)(k0 + p) * npad + e], x[k0 + p], a);
             x[e] -= a;
         }
@@ -252,7 +254,9 @@ __global__ __launch_bounds__(NT) void batch_solve_kernel(
         // ------------------------------------------------- solve + refinement
         const int64_t vo = sys[s].v_off;
         const float* a0 = A0 + sys[s].a_off;
-        float* x32 = xs + vo;
+        // the FP32 working vector of the substitutions lives in LDS (in the
+        // dead strip copies; maxm >= 32 makes room for npad floats there)
+        float* x32 = reinterpret_cast<float*>(TopT);
         double* x = x64 + vo;
         double* r = rs + vo;
         const double* b = b64 + vo;
@@ -262,15 +266,28 @@ __global__ __launch_bounds__(NT) void batch_solve_kernel(
         for (int i = t; i < npad; i += NT) x[i] = (double)x32[i];
         __syncthreads();
         double anorm = 0.0, berr = 0.0;
+        // x is staged in LDS (the strip copies are dead after the
+        // factorisation) so the residual's inner loop streams only the matrix,
+        // 32 rows per unrolled chunk with their loads in flight together; the
+        // sums still run over j in increasing order
+        double* xsh = reinterpret_cast<double*>(Left);
         int it = 0;
         for (;; ++it) {
             double rmax = 0.0, amax = 0.0, xmax = 0.0, bmax = 0.0;
+            for (int i = t; i < npad; i += NT) xsh[i] = x[i];
+            __syncthreads();
             for (int i = t; i < npad; i += NT) {
                 double acc = 0.0, sa = 0.0;
-                for (int j = 0; j < npad; ++j) {
-                    const double v = (double)a0[(int64_t)j * npad + i];
-                    acc = fma(v, x[j], acc);
-                    sa += fabs(v);
+                for (int j0 = 0; j0 < npad; j0 += 32) {
+                    float v32[32];
+#pragma unroll
+                    for (int jj = 0; jj < 32; ++jj) v32[jj] = a0[(int64_t)(j0 + jj) * npad + i];
+#pragma unroll
+                    for (int jj = 0; jj < 32; ++jj) {
+                        const double v = (double)v32[jj];
+                        acc = fma(v, xsh[j0 + jj], acc);
+                        sa += fabs(v);
+                    }
                 }
                 const double ri = b[i] - acc;
                 r[i] = ri;
