@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -529,7 +530,8 @@ void hpl_destroy(hpl_ctx* c) {
                         (void*)c->dInner, (void*)c->dStatus, (void*)c->dWork, (void*)c->dOut4,
                         c->dLpT, c->dLpL, (void*)c->dScT, (void*)c->dScL, (void*)c->dX64,
                         (void*)c->dIrR, (void*)c->dIrXs, (void*)c->dIrOut, (void*)c->dRhsB,
-                        (void*)c->dRhsX})
+                        (void*)c->dRhsX, (void*)c->dGmV, (void*)c->dGmW, (void*)c->dGmC,
+                        (void*)c->dGmWork})
             if (p) (void)hipFree(p);
     for (auto& e : c->ev) {
         (void)hipEventDestroy(e.first);
@@ -780,6 +782,195 @@ hpl_status hpl_solve_ir(hpl_ctx* c, uint32_t block, hpl_prec prec, int max_iter,
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
     return st;
+}
+
+namespace {
+
+// z = F^-1 v for an FP64 vector v (FP32 substitution with the resident
+// factor, result promoted); xs is the FP32 scratch of length n.
+hpl_status precond_apply(hpl_ctx* c, const double* v, double* z) {
+    const int64_t n = (int64_t)c->n;
+    const int B = (int)c->B;
+    const double one = 1.0;
+    HIPCHK(c, launch_d2f_multi(v, n, c->dIrXs, n, n, 1, &one, c->s0));
+    HIPCHK(c, launch_gesl_multi(c->dA, (int64_t)c->lda, n, B, c->dInv, 3 * (B / 32) * 1024,
+                                c->dIrXs, n, 1, c->s0));
+    HIPCHK(c, launch_axpy_multi(z, n, c->dIrXs, n, n, 1, &one, 1, c->s0));
+    return HPL_SUCCESS;
+}
+
+// Givens rotation (c, s) with c a + s b = r, -s a + c b = 0, following the
+// reference's rotmat (LINPACK/src/host/gmres.c:29-43).
+void givens(double a, double b, double* cs, double* sn) {
+    if (b == 0.0) {
+        *cs = 1.0;
+        *sn = 0.0;
+    } else if (std::fabs(b) > std::fabs(a)) {
+        const double t = a / b;
+        *sn = 1.0 / std::sqrt(1.0 + t * t);
+        *cs = t * *sn;
+    } else {
+        const double t = b / a;
+        *cs = 1.0 / std::sqrt(1.0 + t * t);
+        *sn = t * *cs;
+    }
+}
+
+}  // namespace
+
+/* GMRES-based refinement: each correction solves M d = r with restarted,
+ * left-preconditioned GMRES in FP64 (the reference's gmres_ref,
+ * LINPACK/src/host/gmres.c:53-232, with the low-precision factor as the
+ * preconditioner and classical Gram-Schmidt applied twice), then x += d. */
+hpl_status hpl_solve_gmres(hpl_ctx* c, uint32_t block, hpl_prec prec, int restart, int max_iter,
+                           double tol, hpl_ir_result* out) {
+    if (!c || !out) return HPL_ERR_INVALID_ARG;
+    if (!c->have_orig) return fail(c, HPL_ERR_STATE, "hpl_solve_gmres needs hpl_keep_original(1)");
+    if (max_iter < 0 || restart < 1 || restart > 64)
+        return fail(c, HPL_ERR_INVALID_ARG, "restart must be in [1,64] and max_iter >= 0");
+    HIPCHK(c, hipSetDevice(c->device));
+    const int64_t n = (int64_t)c->n;
+    const int m = restart;
+    hpl_status st = ensure(c, c->dX64, c->capIR, n * sizeof(double));
+    if (!st) st = ensure(c, c->dGmV, c->capGmV, (size_t)(m + 1) * n * sizeof(double));
+    if (!st) st = ensure(c, c->dGmW, c->capGmW, (size_t)n * sizeof(double));
+    if (!st) st = ensure(c, c->dGmC, c->capGmC, (size_t)(2 * m + 2) * sizeof(double));
+    if (!st)
+        st = ensure(c, c->dGmWork, c->capGmWork,
+                    (size_t)gmres_work_doubles(n, m + 1) * sizeof(double));
+    if (!st) {
+        const hplx::IrPlan p = hplx::plan_ir(c->n, 1);
+        st = ensure(c, c->dWork, c->capWork, p.work);
+        if (!st) st = ensure(c, c->dIrR, c->capIrR, p.r);
+        if (!st) st = ensure(c, c->dIrXs, c->capIrXs, p.xs);
+        if (!st) st = ensure(c, c->dIrOut, c->capIrOut, p.out);
+    }
+    if (st) return st;
+    HIPCHK(c, hipMemcpyAsync(c->dA, c->dA0, n * c->lda * sizeof(float), hipMemcpyDeviceToDevice,
+                             c->s0));
+    hipEvent_t t0, t1;
+    HIPCHK(c, hipEventCreate(&t0));
+    if (hipEventCreate(&t1) != hipSuccess) {
+        (void)hipEventDestroy(t0);
+        return fail(c, HPL_ERR_HIP, "hipEventCreate failed");
+    }
+    struct EventGuard {
+        hipEvent_t a, b;
+        ~EventGuard() {
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+        }
+    } guard{t0, t1};
+    HIPCHK(c, hipEventRecord(t0, c->s0));
+    hpl_timing tg{};
+    st = hpl_gefa_dev(c, block, prec, &tg);
+    if (st) return st;
+    double* V = c->dGmV;
+    double* W = c->dGmW;
+    double* dh = c->dGmC;  // coefficients: [0, m+1) pass 1 / solution, [m+1, 2m+2) pass 2
+    double* work = c->dGmWork;
+    // x0 = F^-1 b
+    st = precond_apply(c, c->db64, c->dX64);
+    if (st) return st;
+    std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), h1(m + 1), h2(m + 1),
+        y(m);
+    double hres[4];
+    int inner_total = 0, outer = 0;
+    double berr = 0.0;
+    for (;; ++outer) {
+        HIPCHK(c, launch_residual_multi(c->dA0, (int64_t)c->lda, n, c->dX64, n, c->db64, n, 1,
+                                        c->dWork, c->dIrR, n, c->dIrOut, c->s0));
+        HIPCHK(c, hipMemcpyAsync(hres, c->dIrOut, sizeof hres, hipMemcpyDeviceToHost, c->s0));
+        HIPCHK(c, hipStreamSynchronize(c->s0));
+        berr = hres[0] / (hres[1] * hres[2] + hres[3]);
+        if (berr <= tol || outer >= max_iter || !(hres[0] > 0.0) || !(hres[0] < HUGE_VAL)) break;
+        // GMRES on F^-1 M d = F^-1 r, d0 = 0: V0 = z / beta
+        st = precond_apply(c, c->dIrR, W);
+        if (st) return st;
+        HIPCHK(c, launch_multi_dot(W, n, 1, W, n, dh, work, c->s0));
+        double beta2 = 0.0;
+        HIPCHK(c, hipMemcpyAsync(&beta2, dh, sizeof(double), hipMemcpyDeviceToHost, c->s0));
+        HIPCHK(c, hipStreamSynchronize(c->s0));
+        const double beta = std::sqrt(beta2);
+        if (!(beta > 0.0) || !(beta < HUGE_VAL)) break;
+        HIPCHK(c, launch_scale_copy(V, W, 1.0 / beta, n, c->s0));
+        std::fill(H.begin(), H.end(), 0.0);
+        std::fill(g.begin(), g.end(), 0.0);
+        g[0] = beta;
+        int k = 0;
+        for (int i = 0; i < m; ++i) {
+            // W = F^-1 M V_i
+            HIPCHK(c, launch_matvec_f64(c->dA0, (int64_t)c->lda, n, V + (int64_t)i * n, W, work,
+                                        c->s0));
+            st = precond_apply(c, W, W);
+            if (st) return st;
+            // classical Gram-Schmidt, twice: h = V^T W, W -= V h
+            HIPCHK(c, launch_multi_dot(V, n, i + 1, W, n, dh, work, c->s0));
+            HIPCHK(c, launch_multi_axpy(W, 1.0, -1.0, V, n, i + 1, dh, n, c->s0));
+            HIPCHK(c, launch_multi_dot(V, n, i + 1, W, n, dh + m + 1, work, c->s0));
+            HIPCHK(c, launch_multi_axpy(W, 1.0, -1.0, V, n, i + 1, dh + m + 1, n, c->s0));
+            double nw2 = 0.0;
+            HIPCHK(c, hipMemcpyAsync(h1.data(), dh, (i + 1) * sizeof(double),
+                                     hipMemcpyDeviceToHost, c->s0));
+            HIPCHK(c, hipMemcpyAsync(h2.data(), dh + m + 1, (i + 1) * sizeof(double),
+                                     hipMemcpyDeviceToHost, c->s0));
+            HIPCHK(c, launch_multi_dot(W, n, 1, W, n, dh + 2 * m + 1, work, c->s0));
+            HIPCHK(c, hipMemcpyAsync(&nw2, dh + 2 * m + 1, sizeof(double), hipMemcpyDeviceToHost,
+                                     c->s0));
+            HIPCHK(c, hipStreamSynchronize(c->s0));
+            const double hn = std::sqrt(nw2);
+            double* col = &H[(size_t)i * (m + 1)];  // column i of H, rows 0..i+1
+            for (int q = 0; q <= i; ++q) col[q] = h1[q] + h2[q];
+            col[i + 1] = hn;
+            if (hn > 0.0 && i + 1 < m + 1)
+                HIPCHK(c, launch_scale_copy(V + (int64_t)(i + 1) * n, W, 1.0 / hn, n, c->s0));
+            // previous rotations, then the new one (reference: gmres.c:145-162)
+            for (int q = 0; q < i; ++q) {
+                const double tq = cs[q] * col[q] + sn[q] * col[q + 1];
+                col[q + 1] = -sn[q] * col[q] + cs[q] * col[q + 1];
+                col[q] = tq;
+            }
+            givens(col[i], col[i + 1], &cs[i], &sn[i]);
+            const double gi = cs[i] * g[i];
+            g[i + 1] = -sn[i] * g[i];
+            g[i] = gi;
+            col[i] = cs[i] * col[i] + sn[i] * col[i + 1];
+            col[i + 1] = 0.0;
+            k = i + 1;
+            // the preconditioned system is solved to the FP32 accuracy of the
+            // preconditioner; the outer loop measures the true residual
+            if (std::fabs(g[i + 1]) <= 1e-10 * beta || !(hn > 0.0)) break;
+        }
+        // y = H(0:k, 0:k)^-1 g, x += V y
+        for (int q = k - 1; q >= 0; --q) {
+            double sum = g[q];
+            for (int p2 = q + 1; p2 < k; ++p2) sum -= H[(size_t)p2 * (m + 1) + q] * y[p2];
+            y[q] = sum / H[(size_t)q * (m + 1) + q];
+        }
+        HIPCHK(c, hipMemcpyAsync(dh, y.data(), k * sizeof(double), hipMemcpyHostToDevice, c->s0));
+        HIPCHK(c, launch_multi_axpy(c->dX64, 1.0, 1.0, V, n, k, dh, n, c->s0));
+        inner_total += k;
+    }
+    float ms = 0.f;
+    HIPCHK(c, hipEventRecord(t1, c->s0));
+    HIPCHK(c, hipEventSynchronize(t1));
+    HIPCHK(c, hipEventElapsedTime(&ms, t0, t1));
+    out->iterations = inner_total;
+    out->converged = berr <= tol ? 1 : 0;
+    out->backward_error = berr;
+    out->residual_inf = hres[0];
+    out->norm_a = hres[1];
+    out->norm_x = hres[2];
+    out->norm_b = hres[3];
+    out->gefa_s = tg.gefa_s;
+    out->total_s = ms * 1e-3;
+    c->factored = true;
+    if (!out->converged)
+        return fail(c, HPL_ERR_NOT_CONVERGED,
+                    "GMRES refinement stopped after %d outer / %d inner iterations at backward "
+                    "error %.3e (tol %.3e)",
+                    outer, inner_total, berr, tol);
+    return HPL_SUCCESS;
 }
 
 /* Solve nrhs further right-hand sides with the resident factor and refine

@@ -76,6 +76,13 @@ struct hpl_ctx {
     float* dIrXs = nullptr;
     double* dIrOut = nullptr;
     size_t capIrR = 0, capIrXs = 0, capIrOut = 0;
+    // GMRES refinement: Krylov basis (restart + 1 vectors of n), work vector,
+    // basis coefficients and reduction scratch
+    double* dGmV = nullptr;
+    double* dGmW = nullptr;
+    double* dGmC = nullptr;
+    double* dGmWork = nullptr;
+    size_t capGmV = 0, capGmW = 0, capGmC = 0, capGmWork = 0;
     // right-hand sides and solutions of hpl_solve_rhs64
     double* dRhsB = nullptr;
     double* dRhsX = nullptr;

@@ -100,6 +100,20 @@ hipError_t launch_axpy_multi(double* X, int64_t ldx, const float* D, int64_t ldd
 hipError_t launch_pad_identity(float* A, int64_t lda, int64_t n, int64_t npad, hipStream_t s);
 hipError_t launch_fill_f64(double* p, int64_t count, double v, hipStream_t s);
 
+// FP64 Krylov-basis kernels of the GMRES refinement (gmres.hip); work holds
+// gmres_work_doubles(n, kmax) doubles for up to kmax basis vectors
+int64_t gmres_work_doubles(int64_t n, int kmax);
+// y = M x with M(i, j) = A[j][i] (FP32 matrix, FP64 vectors)
+hipError_t launch_matvec_f64(const float* A, int64_t lda, int64_t n, const double* x, double* y,
+                             double* work, hipStream_t s);
+// out[k] = V[k] . w for k < kc (basis vector k at V + k * ldv)
+hipError_t launch_multi_dot(const double* V, int64_t ldv, int kc, const double* w, int64_t n,
+                            double* out, double* work, hipStream_t s);
+// y = alpha y + beta sum_k c[k] V[k]  (c on the device)
+hipError_t launch_multi_axpy(double* y, double alpha, double beta, const double* V, int64_t ldv,
+                             int kc, const double* c, int64_t n, hipStream_t s);
+hipError_t launch_scale_copy(double* dst, const double* src, double a, int64_t n, hipStream_t s);
+
 // batched solves (batch.hip)
 struct BatchSystem {
     int64_t a_off;  // matrix offset (elements) in the matrix pool

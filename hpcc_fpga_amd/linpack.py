@@ -127,6 +127,7 @@ def load_library():
         "hpl_download_solution64": ([P, P], i32),
         "hpl_upload_rhs64": ([P, P], i32),
         "hpl_solve_rhs64": ([P, i32, P, u64, P, u64, i32, dbl, P], i32),
+        "hpl_solve_gmres": ([P, u32, i32, i32, i32, dbl, C.POINTER(_IRResult)], i32),
         "hpl_serve_config_default": ([C.POINTER(_ServeConfig)], None),
         "hpl_serve_create": ([C.POINTER(_ServeConfig), C.POINTER(P)], i32),
         "hpl_serve_destroy": ([P], None),
@@ -279,6 +280,19 @@ class Context:
             tol = 1e-14 * self.n
         res = _IRResult()
         st = self._L.hpl_solve_ir(self._c, block, PRECISIONS[prec], max_iter, tol, C.byref(res))
+        if st != HPL_SUCCESS and (check or st != 7):
+            self._check(st)
+        return {f: getattr(res, f) for f, _ in _IRResult._fields_}
+
+    def solve_gmres(self, block: int, prec: str = "bf16", restart: int = 20, max_iter: int = 10,
+                    tol: float = None, check: bool = True) -> dict:
+        """Mixed-precision factorisation + GMRES-based FP64 refinement (see
+        hpl_solve_gmres); ``iterations`` counts Arnoldi steps."""
+        if tol is None:
+            tol = 1e-14 * self.n
+        res = _IRResult()
+        st = self._L.hpl_solve_gmres(self._c, block, PRECISIONS[prec], restart, max_iter, tol,
+                                     C.byref(res))
         if st != HPL_SUCCESS and (check or st != 7):
             self._check(st)
         return {f: getattr(res, f) for f, _ in _IRResult._fields_}

@@ -151,6 +151,17 @@ typedef struct hpl_ir_result {
 } hpl_ir_result;
 hpl_status hpl_solve_ir(hpl_ctx* ctx, uint32_t block, hpl_prec prec, int max_iter, double tol,
                         hpl_ir_result* result);
+/* As hpl_solve_ir, but each correction M d = r is solved with restarted,
+ * left-preconditioned GMRES(restart) in FP64 (restart 1..64), the
+ * low-precision factor applied as the preconditioner through the FP32
+ * substitution, before x += d.  This follows the reference's test-only
+ * FGMRES (LINPACK/src/host/gmres.c:53-232: Arnoldi, Givens rotations of
+ * rotmat :29-43, triangular solve of the Hessenberg system) with classical
+ * Gram-Schmidt applied twice; it converges where the classic iteration's
+ * contraction is too weak (FP8 factors of poorly conditioned matrices).
+ * result->iterations counts Arnoldi steps over all max_iter outer steps. */
+hpl_status hpl_solve_gmres(hpl_ctx* ctx, uint32_t block, hpl_prec prec, int restart,
+                           int max_iter, double tol, hpl_ir_result* result);
 /* Solve nrhs (1..16) further right-hand sides with the resident factor (of
  * hpl_solve_ir, or of hpl_gefa_dev on a kept original) and refine each column
  * as above: b and x are n x nrhs FP64, column j at b + j*ldb / x + j*ldx;

@@ -110,3 +110,51 @@ def test_refinement_fp32_factor(ctx):
     res = ctx.solve_ir(B, "fp32", max_iter=10)
     assert res["converged"] == 1 and res["iterations"] <= 3
     ctx.keep_original(False)
+
+
+@pytest.mark.parametrize("prec,d", [("bf16", 1.1), ("fp8", 1.01)])
+def test_gmres_refinement_matches_fp64_and_reference_gmres(ctx, prec, d):
+    """hpl_solve_gmres reaches 1e-14 n; its x agrees with the FP64 oracle
+    solution and with the reference's own gmres_ref (LINPACK/src/host/gmres.c,
+    compiled unchanged into oracle/_ref) run with an FP64 LU preconditioner."""
+    n, B = 1024, 128
+    print(f"gmres {prec} d={d}", flush=True)
+    ctx.keep_original(True)
+    ctx.generate(n, 1, 11, d)
+    res = ctx.solve_gmres(B, prec, restart=20, max_iter=10)
+    assert res["converged"] == 1 and res["backward_error"] <= 1e-14 * n
+    x = ctx.download_solution64()
+    A, _, b64, _ = O.generate_sized(n, 11, 1, d)
+    F = O.gefa_nopvt_f64(A.astype(np.float64))
+    x_ref = O.gesl_nopvt_f64(F, b64)
+    assert np.max(np.abs(x - x_ref)) / np.max(np.abs(x_ref)) <= 1e-10
+    assert O.backward_error(A, x, b64) <= 1e-14 * n
+    R = O.ref_lib()
+    if R is not None:
+        L, U = O.decode_factor(F)
+        LU = np.ascontiguousarray((np.tril(L, -1) + U).T)
+        xg = np.zeros(n)
+        R.gmres_ref(n, np.ascontiguousarray(A.astype(np.float64)), n, xg, b64.copy(), LU, n, 20,
+                    10, 1e-14)
+        assert np.max(np.abs(x - xg)) / np.max(np.abs(xg)) <= 1e-10
+    ctx.keep_original(False)
+
+
+def test_gmres_needs_fewer_corrections_than_classic_with_fp8(ctx):
+    """Reference generator (positive entries, coherent trailing updates) with
+    an FP8 factor: both refinements reach 1e-14 n; GMRES takes at most as many
+    outer corrections (matrix passes through the residual) as the classic one."""
+    n, B = 2048, 256
+    print("gmres vs classic fp8", flush=True)
+    A, b, _, _ = O.generate_ref(n, seed=3)
+    ctx.keep_original(True)
+    ctx.upload(A)
+    ctx.upload_rhs64(b.astype(np.float64))
+    classic = ctx.solve_ir(B, "fp8", max_iter=60, check=False)
+    gm = ctx.solve_gmres(B, "fp8", restart=20, max_iter=10, check=False)
+    print(f"  classic: {classic['iterations']} it, berr {classic['backward_error']:.2e}; "
+          f"gmres: {gm['iterations']} Arnoldi steps, berr {gm['backward_error']:.2e}", flush=True)
+    assert gm["converged"] == 1 and gm["backward_error"] <= 1e-14 * n
+    x = ctx.download_solution64()
+    assert O.backward_error(A, x, b.astype(np.float64)) <= 1e-14 * n
+    ctx.keep_original(False)
