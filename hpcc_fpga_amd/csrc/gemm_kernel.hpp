@@ -47,7 +47,11 @@ struct Region {
     int M, N;
 };
 
-template <int BM, int BN, int BK, int WM, int WN, int MINB, bool CPRE, bool TRANS = false>
+// DMA: stage the slabs with global_load_lds (16 bytes per lane, written to LDS
+// lane-linearly at a wave-uniform base), which needs no staging registers and
+// no LDS write instructions; each wave moves whole 1 KiB pieces of a slab.
+template <int BM, int BN, int BK, int WM, int WN, int MINB, bool CPRE, bool TRANS = false,
+          bool DMA = false>
 __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64, MINB) void update_kernel(
     float* __restrict__ C, int64_t ldc, const float* __restrict__ At, int64_t lda,
     const float* __restrict__ Bt, int64_t ldb, int M, int N, int K, Region r2) {
@@ -84,7 +88,29 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64, MINB) void update_kerne
     const int lr = lane & 31, lh = lane >> 5;
     const bool full = (m0 + BM <= M) && (n0 + BN <= N);
 
-    f32x4 ra[A4], rb[B4];
+    f32x4 ra[DMA ? 1 : A4], rb[DMA ? 1 : B4];
+    // DMA: float4 index f = tid + NT * s of a slab lands at LDS float 4 * f,
+    // i.e. wave w's lanes fill floats [256 (w + NT/64 s), +256) in order
+    auto dma_slab = [&](int k0, int buf) {
+#pragma unroll
+        for (int s = 0; s < A4; ++s) {
+            const int f = tid + NT * s;
+            const int kk = f / (BM / 4), cc = min((f % (BM / 4)) * 4, M - m0 - 4);
+            __builtin_amdgcn_global_load_lds(
+                At + (int64_t)(k0 + kk) * lda + m0 + cc,
+                (__attribute__((address_space(3))) void*)(&As[buf][0][0] + 256 * (wave + (NT / 64) * s)),
+                16, 0, 0);
+        }
+#pragma unroll
+        for (int s = 0; s < B4; ++s) {
+            const int f = tid + NT * s;
+            const int kk = f / (BN / 4), cc = min((f % (BN / 4)) * 4, N - n0 - 4);
+            __builtin_amdgcn_global_load_lds(
+                Bt + (int64_t)(k0 + kk) * ldb + n0 + cc,
+                (__attribute__((address_space(3))) void*)(&Bs[buf][0][0] + 256 * (wave + (NT / 64) * s)),
+                16, 0, 0);
+        }
+    };
     auto load_slab = [&](int k0) {
 #pragma unroll
         for (int s = 0; s < A4; ++s) {
@@ -176,8 +202,13 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64, MINB) void update_kerne
                                   : mfma32(a[slot][i], b[slot][j], acc[i][j]);
     };
 
-    load_slab(0);
-    store_slab(0);
+    if constexpr (DMA) {
+        dma_slab(0, 0);
+        __builtin_amdgcn_s_waitcnt(0);
+    } else {
+        load_slab(0);
+        store_slab(0);
+    }
     __syncthreads();
     const int nk = K / BK;
     constexpr int KP = BK / 2;  // k-pairs per slab
@@ -187,7 +218,11 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64, MINB) void update_kerne
         const int buf = kc & 1;
         const bool more = kc + 1 < nk;
         // the last slab reloads itself (in bounds, unused) to stay branch-free
-        load_slab((more ? kc + 1 : kc) * BK);
+        if constexpr (DMA) {
+            if (more) dma_slab((kc + 1) * BK, buf ^ 1);
+        } else {
+            load_slab((more ? kc + 1 : kc) * BK);
+        }
         if (kc == kc_c) load_c();
         // k-pairs 0 .. KP-2: read the next operands, then multiply the current
 #pragma unroll
@@ -202,7 +237,10 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64, MINB) void update_kerne
         }
         // last k-pair: hand over to the next slab while its MFMAs run
         if (more) {
-            store_slab(buf ^ 1);
+            if constexpr (DMA)
+                __builtin_amdgcn_s_waitcnt(0);
+            else
+                store_slab(buf ^ 1);
             __syncthreads();
             read_ops(buf ^ 1, 0, KP & 1);
         }
@@ -227,7 +265,8 @@ __global__ __launch_bounds__((BM / WM) * (BN / WN) * 64, MINB) void update_kerne
 }
 
 // One launch over region (C, At, Bt, M, N) and the optional second region r2.
-template <int BM, int BN, int BK, int WM, int WN, int MINB, bool CPRE, bool TRANS = false>
+template <int BM, int BN, int BK, int WM, int WN, int MINB, bool CPRE, bool TRANS = false,
+          bool DMA = false>
 inline hipError_t launch2(float* C, int64_t ldc, const float* At, int64_t lda, const float* Bt,
                           int64_t ldb, int M, int N, int K, Region r2, hipStream_t s) {
     if (r2.M <= 0 || r2.N <= 0) r2.M = r2.N = 0;
@@ -244,15 +283,16 @@ inline hipError_t launch2(float* C, int64_t ldc, const float* At, int64_t lda, c
     constexpr int NT = (BM / WM) * (BN / WN) * 64;
     const int tiles = ((M + BM - 1) / BM) * ((N + BN - 1) / BN) +
                       ((r2.M + BM - 1) / BM) * ((r2.N + BN - 1) / BN);
-    update_kernel<BM, BN, BK, WM, WN, MINB, CPRE, TRANS>
+    update_kernel<BM, BN, BK, WM, WN, MINB, CPRE, TRANS, DMA>
         <<<tiles, NT, 0, s>>>(C, ldc, At, lda, Bt, ldb, M, N, K, r2);
     return hipGetLastError();
 }
 
-template <int BM, int BN, int BK, int WM, int WN, int MINB, bool CPRE, bool TRANS = false>
+template <int BM, int BN, int BK, int WM, int WN, int MINB, bool CPRE, bool TRANS = false,
+          bool DMA = false>
 inline hipError_t launch(float* C, int64_t ldc, const float* At, int64_t lda, const float* Bt,
                          int64_t ldb, int M, int N, int K, hipStream_t s) {
-    return launch2<BM, BN, BK, WM, WN, MINB, CPRE, TRANS>(C, ldc, At, lda, Bt, ldb, M, N, K,
+    return launch2<BM, BN, BK, WM, WN, MINB, CPRE, TRANS, DMA>(C, ldc, At, lda, Bt, ldb, M, N, K,
                                                           Region{nullptr, nullptr, nullptr, 0, 0},
                                                           s);
 }
