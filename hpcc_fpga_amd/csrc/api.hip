@@ -384,7 +384,21 @@ WsPlan plan_factor_ws(uint64_t n, uint32_t B, int prec, int depth) {
     w.lp = prec == HPL_PREC_FP32 ? 0 : 2 * (size_t)depth * n * B * lp_elem_bytes(prec);
     w.sc = prec == HPL_PREC_FP8 ? 2 * (size_t)depth * (n / 32) * sizeof(float) : 0;
     w.status = 64;
+    w.binv = B <= 256 ? 2 * (size_t)n * B * sizeof(float) : 0;
     return w;
+}
+
+hpl_status ensure_block_inverses(hpl_ctx* c) {
+    if (c->binv_ready) return HPL_SUCCESS;
+    const uint32_t B = c->B;
+    if (B > 256) return HPL_SUCCESS;  // sequential diagonal solves
+    const hplx::WsPlan w = plan_factor_ws(c->n, B, c->prec, 1);
+    hpl_status st = ensure(c, c->dBInv, c->capBInv, w.binv);
+    if (st) return st;
+    HIPCHK(c, launch_block_inverse(c->dA, (int64_t)c->lda, (int64_t)c->n, (int)B, c->dInv,
+                                   3 * (B / 32) * 1024, c->dBInv, c->s0));
+    c->binv_ready = true;
+    return HPL_SUCCESS;
 }
 
 IrPlan plan_ir(uint64_t n, int nrhs) {
@@ -419,6 +433,8 @@ hpl_status ir_solve(hpl_ctx* c, int nrhs, const double* dB, double* dX, int max_
     if (st) return st;
     st = ensure(c, c->dIrOut, c->capIrOut, p.out);
     if (st) return st;
+    st = ensure_block_inverses(c);
+    if (st) return st;
     const int B = (int)c->B;
     const int64_t inv_stride = 3 * (B / 32) * 1024;
     double sc_in[16], sc_out[16], h[64];
@@ -430,7 +446,7 @@ hpl_status ir_solve(hpl_ctx* c, int nrhs, const double* dB, double* dX, int max_
     // x0 = F^-1 b in FP32, promoted
     HIPCHK(c, launch_d2f_multi(dB, n, c->dIrXs, n, n, nrhs, sc_in, c->s0));
     HIPCHK(c, launch_gesl_multi(c->dA, (int64_t)c->lda, n, B, c->dInv, inv_stride, c->dIrXs, n,
-                                nrhs, c->s0));
+                                nrhs, c->s0, c->binv_ready ? c->dBInv : nullptr));
     HIPCHK(c, launch_axpy_multi(dX, n, c->dIrXs, n, n, nrhs, sc_out, 1, c->s0));
     for (int it = 0;; ++it) {
         HIPCHK(c, launch_residual_multi(c->dA0, (int64_t)c->lda, n, dX, n, dB, n, nrhs, c->dWork,
@@ -464,7 +480,7 @@ hpl_status ir_solve(hpl_ctx* c, int nrhs, const double* dB, double* dX, int max_
         // columns get scale 0 and stay unchanged)
         HIPCHK(c, launch_d2f_multi(c->dIrR, n, c->dIrXs, n, n, nrhs, sc_in, c->s0));
         HIPCHK(c, launch_gesl_multi(c->dA, (int64_t)c->lda, n, B, c->dInv, inv_stride, c->dIrXs,
-                                    n, nrhs, c->s0));
+                                    n, nrhs, c->s0, c->binv_ready ? c->dBInv : nullptr));
         HIPCHK(c, launch_axpy_multi(dX, n, c->dIrXs, n, n, nrhs, sc_out, 0, c->s0));
     }
     for (int j = 0; j < nrhs; ++j)
@@ -531,7 +547,7 @@ void hpl_destroy(hpl_ctx* c) {
                         c->dLpT, c->dLpL, (void*)c->dScT, (void*)c->dScL, (void*)c->dX64,
                         (void*)c->dIrR, (void*)c->dIrXs, (void*)c->dIrOut, (void*)c->dRhsB,
                         (void*)c->dRhsX, (void*)c->dGmV, (void*)c->dGmW, (void*)c->dGmC,
-                        (void*)c->dGmWork})
+                        (void*)c->dGmWork, (void*)c->dBInv})
             if (p) (void)hipFree(p);
     for (auto& e : c->ev) {
         (void)hipEventDestroy(e.first);
@@ -634,6 +650,7 @@ hpl_status hpl_gefa_dev(hpl_ctx* c, uint32_t block, hpl_prec prec, hpl_timing* t
     if (st) return st;
     HIPCHK(c, hipMemsetAsync(c->dStatus, 0, sizeof(int), c->s0));
     c->factored = false;
+    c->binv_ready = false;
     HIPCHK(c, hipEventRecord(c->e0, c->s0));
     st = gefa_schedule(c, block);
     if (st) return st;
@@ -652,9 +669,11 @@ hpl_status hpl_gesl_dev(hpl_ctx* c, hpl_timing* t) {
     if (!c->factored) return fail(c, HPL_ERR_STATE, "hpl_gesl_dev before a successful GEFA");
     HIPCHK(c, hipSetDevice(c->device));
     HIPCHK(c, hipEventRecord(c->e0, c->s0));
+    hpl_status st = hplx::ensure_block_inverses(c);
+    if (st) return st;
     HIPCHK(c, hipMemcpyAsync(c->dx, c->db, c->n * sizeof(float), hipMemcpyDeviceToDevice, c->s0));
     HIPCHK(c, launch_gesl(c->dA, (int64_t)c->lda, (int64_t)c->n, (int)c->B, c->dInv,
-                          3 * (c->B / 32) * 1024, c->dx, c->s0));
+                          3 * (c->B / 32) * 1024, c->dx, c->s0, c->binv_ready ? c->dBInv : nullptr));
     HIPCHK(c, hipEventRecord(c->e1, c->s0));
     HIPCHK(c, hipStreamSynchronize(c->s0));
     if (t) t->gesl_s = elapsed_s(c);
@@ -721,6 +740,7 @@ hpl_status hpl_gesl(hpl_ctx* c, const float* lu, float* b, uint64_t n, uint64_t 
                               3 * (B / 32) * 1024, c->s0));
     c->B = B;
     c->factored = true;
+    c->binv_ready = false;
     st = hpl_gesl_dev(c, t);
     if (st) return st;
     HIPCHK(c, hipMemcpy(b, c->dx, n * sizeof(float), hipMemcpyDeviceToHost));
@@ -792,9 +812,11 @@ hpl_status precond_apply(hpl_ctx* c, const double* v, double* z) {
     const int64_t n = (int64_t)c->n;
     const int B = (int)c->B;
     const double one = 1.0;
+    hpl_status st = hplx::ensure_block_inverses(c);
+    if (st) return st;
     HIPCHK(c, launch_d2f_multi(v, n, c->dIrXs, n, n, 1, &one, c->s0));
     HIPCHK(c, launch_gesl_multi(c->dA, (int64_t)c->lda, n, B, c->dInv, 3 * (B / 32) * 1024,
-                                c->dIrXs, n, 1, c->s0));
+                                c->dIrXs, n, 1, c->s0, c->binv_ready ? c->dBInv : nullptr));
     HIPCHK(c, launch_axpy_multi(z, n, c->dIrXs, n, n, 1, &one, 1, c->s0));
     return HPL_SUCCESS;
 }

@@ -55,7 +55,7 @@ class HipExecutor : public hplserve::Executor {
         p.npad = (n + p.block - 1) / p.block * p.block;
         p.lda = hplx::device_lda(p.npad);
         const hplx::WsPlan w = hplx::plan_factor_ws(p.npad, p.block, cfg_.prec, 1);
-        p.factor_bytes = 2 * al(p.npad * p.lda * sizeof(float)) + al(w.inv);
+        p.factor_bytes = 2 * al(p.npad * p.lda * sizeof(float)) + al(w.inv) + al(w.binv);
         return p;
     }
 
@@ -162,8 +162,11 @@ class HipExecutor : public hplserve::Executor {
         c->capA0 = abytes;
         c->dA = reinterpret_cast<float*>(mem + abytes);
         c->capA = abytes;
+        const hplx::WsPlan w = hplx::plan_factor_ws(np, p.block, f->prec, 1);
         c->dInv = reinterpret_cast<float*>(mem + 2 * abytes);
-        c->capInv = p.factor_bytes - 2 * abytes;
+        c->capInv = w.inv;
+        c->dBInv = reinterpret_cast<float*>(mem + 2 * abytes + al(w.inv));
+        c->capBInv = w.binv;
         c->keep_orig = c->have_orig = true;
         hipStream_t s0 = c->s0;
         if (job.fresh) {
@@ -183,9 +186,12 @@ class HipExecutor : public hplserve::Executor {
             const hpl_status st = hpl_gefa_dev(c, p.block, f->prec, nullptr);
             if (st) return fail(job, c, st, c->err.c_str());
         } else {
+            // a resident factor: its block inverses were computed by the
+            // request that factorised it
             c->B = p.block;
             c->prec = f->prec;
             c->factored = true;
+            c->binv_ready = true;
         }
         const int nr = rq.nrhs;
         hipError_t e = hipSuccess;

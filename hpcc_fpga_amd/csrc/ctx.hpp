@@ -54,6 +54,9 @@ struct hpl_ctx {
     size_t capDT = 0;
     float* dInv = nullptr;      // per block: 3 * (B/32) inverses of 32 x 32
     size_t capInv = 0;
+    float* dBInv = nullptr;     // full inverses of the diagonal blocks (substitution), N x 2 x B x B
+    size_t capBInv = 0;
+    bool binv_ready = false;    // dBInv holds the inverses of the current factor
     float* dInner = nullptr;    // recursion scratch for blocks > 128
     size_t capInner = 0;
     void* dLpT = nullptr;       // low-precision k-contiguous top-strip copies, 2*depth slots of n x B
@@ -107,7 +110,8 @@ hpl_status init_streams(hpl_ctx* c);
 // with the given trailing-update precision and update depth.
 struct WsPlan {
     size_t lt, dt, inner, lp, sc, status;
-    size_t inv;  // per-block inverses (kept with the factor)
+    size_t inv;   // per-block 32 x 32 inverses (kept with the factor)
+    size_t binv;  // full diagonal-block inverses for the substitution (kept with the factor)
 };
 WsPlan plan_factor_ws(uint64_t n, uint32_t B, int prec, int depth);
 // Device bytes of the refinement buffers for nrhs columns of order n.
@@ -124,5 +128,9 @@ hpl_status ir_solve(hpl_ctx* c, int nrhs, const double* dB, double* dX, int max_
                     hpl_ir_result* res);
 
 uint64_t device_lda(uint64_t n);
+
+// The full diagonal-block inverses of the resident factor (computed once per
+// factorisation, for blocks up to 256); dBInv stays null for larger blocks.
+hpl_status ensure_block_inverses(hpl_ctx* c);
 
 }  // namespace hplx

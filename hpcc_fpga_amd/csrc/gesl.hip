@@ -15,7 +15,9 @@
 // the other workgroups apply the same solved block to all further entries in
 // parallel.  The diagonal solve works on 32-row sub-blocks: the contribution
 // of solved rows is a dot product spread over the workgroup, and the 32 x 32
-// triangular system is applied as its inverse (computed by the panel prep).
+// triangular system is applied as its inverse (computed by the panel prep);
+// with the full inverses of the diagonal blocks (launch_block_inverse, B <=
+// 256) the diagonal solve is instead one coalesced GEMV.
 //
 // Right-hand sides: column r of X at X + r * ldx.  NR is the compile-time
 // capacity (1, 4, 16), nr <= NR the count in use.
@@ -47,7 +49,8 @@ template <int NR, int SIGN>
 __device__ __forceinline__ void block_gemv(const float* __restrict__ A, int64_t lda, int64_t pbase,
                                            int64_t ebase, int ncols, int B, const float* v,
                                            int vstride, int nr, float* red, int64_t elim,
-                                           float* __restrict__ X, int64_t ldx) {
+                                           float* __restrict__ X, int64_t ldx,
+                                           int64_t eshift = 0) {
     const int t = threadIdx.x;
     const int groups = NT / ncols;
     const int col = t % ncols, g = t / ncols;
@@ -58,7 +61,7 @@ __device__ __forceinline__ void block_gemv(const float* __restrict__ A, int64_t 
     const int64_t e = ebase + 4 * col;
     const bool live = e < elim;  // elim is a multiple of 4
     if (live) {
-        const float* src = A + pbase * lda + e;
+        const float* src = A + pbase * lda + (e - eshift);  // column e - eshift of A
 #pragma unroll 16
         for (int p = g; p < B; p += groups) {
             const f32x4 m = *reinterpret_cast<const f32x4*>(src + (int64_t)p * lda);
@@ -169,6 +172,27 @@ __device__ void diag_solve(const float* __restrict__ A, int64_t lda, int64_t Qof
     }
 }
 
+// Diagonal block at Qoff with its full inverse: X[Qoff..Qoff+B) := W^T z,
+// z the current entries, W[p][i] the stored inverse (SIGN -1: negated).
+template <int NR, int SIGN>
+__device__ void diag_apply(const float* __restrict__ W, int64_t Qoff, int B, float* __restrict__ X,
+                           int64_t ldx, int nr, float* zbuf, float* red) {
+    constexpr int MAXB = gesl_cfg<NR>::MAXB;
+    const int t = threadIdx.x;
+    if (NR == 1) nr = 1;
+    for (int idx = t; idx < nr * B; idx += NT) {
+        const int r = idx / B, e = idx % B;
+        zbuf[r * MAXB + e] = X[r * ldx + Qoff + e];
+        X[r * ldx + Qoff + e] = 0.0f;
+    }
+    __syncthreads();
+    for (int e0 = 0; e0 < B; e0 += kDiagPass) {
+        const int ncols = min(kDiagPass, B - e0) / 4;
+        block_gemv<NR, SIGN>(W, B, 0, Qoff + e0, ncols, B, zbuf, MAXB, nr, red, Qoff + B, X, ldx,
+                             Qoff);
+    }
+}
+
 template <int NR>
 struct gesl_smem {
     float ybuf[NR * gesl_cfg<NR>::MAXB];  // diagonal system being solved
@@ -183,8 +207,9 @@ template <int NR>
 __global__ __launch_bounds__(NT) void gesl_fwd_kernel(const float* __restrict__ A, int64_t lda,
                                                       int64_t n, int B, int q,
                                                       const float* __restrict__ inv,
-                                                      int64_t inv_stride, float* __restrict__ X,
-                                                      int64_t ldx, int nr) {
+                                                      int64_t inv_stride,
+                                                      const float* __restrict__ binv,
+                                                      float* __restrict__ X, int64_t ldx, int nr) {
     constexpr int MAXB = gesl_cfg<NR>::MAXB, CH = gesl_cfg<NR>::CH;
     __shared__ gesl_smem<NR> sm;
     const int t = threadIdx.x;
@@ -208,8 +233,12 @@ __global__ __launch_bounds__(NT) void gesl_fwd_kernel(const float* __restrict__ 
             }
             __syncthreads();
         }
-        diag_solve<NR, true>(A, lda, Q, B, inv + (int64_t)(q + 1) * inv_stride + NQ * 1024, X, ldx,
-                             nr, sm.ybuf, sm.part, sm.zbuf);
+        if (binv)
+            diag_apply<NR, 1>(binv + (int64_t)(q + 1) * 2 * B * B, Q, B, X, ldx, nr, sm.ybuf,
+                              sm.red);
+        else
+            diag_solve<NR, true>(A, lda, Q, B, inv + (int64_t)(q + 1) * inv_stride + NQ * 1024, X,
+                                 ldx, nr, sm.ybuf, sm.part, sm.zbuf);
         return;
     }
     // entries [(q+2)B, n) in chunks of CH
@@ -222,8 +251,9 @@ template <int NR>
 __global__ __launch_bounds__(NT) void gesl_bwd_kernel(const float* __restrict__ A, int64_t lda,
                                                       int B, int q, int N,
                                                       const float* __restrict__ inv,
-                                                      int64_t inv_stride, float* __restrict__ X,
-                                                      int64_t ldx, int nr) {
+                                                      int64_t inv_stride,
+                                                      const float* __restrict__ binv,
+                                                      float* __restrict__ X, int64_t ldx, int nr) {
     constexpr int MAXB = gesl_cfg<NR>::MAXB, CH = gesl_cfg<NR>::CH;
     __shared__ gesl_smem<NR> sm;
     const int t = threadIdx.x;
@@ -246,8 +276,12 @@ __global__ __launch_bounds__(NT) void gesl_bwd_kernel(const float* __restrict__ 
             }
             __syncthreads();
         }
-        diag_solve<NR, false>(A, lda, Q, B, inv + (int64_t)(q - 1) * inv_stride + 2 * NQ * 1024, X,
-                              ldx, nr, sm.ybuf, sm.part, sm.zbuf);
+        if (binv)
+            diag_apply<NR, -1>(binv + (int64_t)(q - 1) * 2 * B * B + (int64_t)B * B, Q, B, X, ldx,
+                               nr, sm.ybuf, sm.red);
+        else
+            diag_solve<NR, false>(A, lda, Q, B, inv + (int64_t)(q - 1) * inv_stride + 2 * NQ * 1024,
+                                  X, ldx, nr, sm.ybuf, sm.part, sm.zbuf);
         return;
     }
     // entries [0, (q-1)B)
@@ -258,19 +292,21 @@ __global__ __launch_bounds__(NT) void gesl_bwd_kernel(const float* __restrict__ 
 
 template <int NR>
 hipError_t gesl_sweeps(const float* A, int64_t lda, int64_t n, int B, const float* inv,
-                       int64_t inv_stride, float* X, int64_t ldx, int nr, hipStream_t s) {
+                       int64_t inv_stride, const float* binv, float* X, int64_t ldx, int nr,
+                       hipStream_t s) {
     constexpr int CH = gesl_cfg<NR>::CH;
     if (B > gesl_cfg<NR>::MAXB) return hipErrorInvalidValue;
     const int N = (int)(n / B);
     for (int q = -1; q <= N - 2; ++q) {
         const int64_t rest = n - (int64_t)(q + 2) * B;
         const int grid = 1 + (q >= 0 ? (int)((rest + CH - 1) / CH) : 0);
-        gesl_fwd_kernel<NR><<<grid, NT, 0, s>>>(A, lda, n, B, q, inv, inv_stride, X, ldx, nr);
+        gesl_fwd_kernel<NR><<<grid, NT, 0, s>>>(A, lda, n, B, q, inv, inv_stride, binv, X, ldx, nr);
     }
     for (int q = N; q >= 1; --q) {
         const int64_t rest = (int64_t)(q - 1) * B;
         const int grid = 1 + (q < N ? (int)((rest + CH - 1) / CH) : 0);
-        gesl_bwd_kernel<NR><<<grid, NT, 0, s>>>(A, lda, B, q, N, inv, inv_stride, X, ldx, nr);
+        gesl_bwd_kernel<NR><<<grid, NT, 0, s>>>(A, lda, B, q, N, inv, inv_stride, binv, X, ldx,
+                                                nr);
     }
     return hipGetLastError();
 }
@@ -278,7 +314,8 @@ hipError_t gesl_sweeps(const float* A, int64_t lda, int64_t n, int B, const floa
 }  // namespace
 
 hipError_t launch_gesl_multi(const float* A, int64_t lda, int64_t n, int B, const float* inv,
-                             int64_t inv_stride, float* X, int64_t ldx, int nrhs, hipStream_t s) {
+                             int64_t inv_stride, float* X, int64_t ldx, int nrhs, hipStream_t s,
+                             const float* binv) {
     if (B > 512 || B < 32 || B % 32 || n % B || nrhs < 1 || (nrhs > 1 && (ldx < n || ldx % 4)))
         return hipErrorInvalidValue;
     for (int r0 = 0; r0 < nrhs; r0 += 16) {
@@ -286,17 +323,17 @@ hipError_t launch_gesl_multi(const float* A, int64_t lda, int64_t n, int B, cons
         float* Xr = X + (int64_t)r0 * ldx;
         hipError_t e;
         if (nr == 1)
-            e = gesl_sweeps<1>(A, lda, n, B, inv, inv_stride, Xr, ldx, 1, s);
+            e = gesl_sweeps<1>(A, lda, n, B, inv, inv_stride, binv, Xr, ldx, 1, s);
         else if (nr <= 4)
-            e = gesl_sweeps<4>(A, lda, n, B, inv, inv_stride, Xr, ldx, nr, s);
+            e = gesl_sweeps<4>(A, lda, n, B, inv, inv_stride, binv, Xr, ldx, nr, s);
         else
-            e = gesl_sweeps<16>(A, lda, n, B, inv, inv_stride, Xr, ldx, nr, s);
+            e = gesl_sweeps<16>(A, lda, n, B, inv, inv_stride, binv, Xr, ldx, nr, s);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
 }
 
 hipError_t launch_gesl(const float* A, int64_t lda, int64_t n, int B, const float* inv,
-                       int64_t inv_stride, float* x, hipStream_t s) {
-    return launch_gesl_multi(A, lda, n, B, inv, inv_stride, x, n, 1, s);
+                       int64_t inv_stride, float* x, hipStream_t s, const float* binv) {
+    return launch_gesl_multi(A, lda, n, B, inv, inv_stride, x, n, 1, s, binv);
 }

@@ -57,13 +57,22 @@ hipError_t launch_gemm_update_fp8(float* C, int64_t ldc, const void* L, int64_t 
                                   int N, int K, int Bblk, hipStream_t s,
                                   const LpRegion2* r2 = nullptr);
 
+// full inverses of the diagonal blocks of a factor for the substitution
+// (trsm.hip): per block k, out + k * 2 * B * B holds OutL (row r = L^-1 e_r)
+// then OutU (OutU[i][c] = -(U^-1)(c, i)); B <= 256
+hipError_t launch_block_inverse(const float* A, int64_t lda, int64_t n, int B, const float* inv,
+                                int64_t inv_stride, float* out, hipStream_t s);
+
 // substitution (gesl.hip)
+// binv: the full diagonal-block inverses of launch_block_inverse (B <= 256),
+// or null for the sequential diagonal solve with the 32 x 32 inverses
 hipError_t launch_gesl(const float* A, int64_t lda, int64_t n, int B, const float* inv,
-                       int64_t inv_stride, float* x, hipStream_t s);
+                       int64_t inv_stride, float* x, hipStream_t s, const float* binv = nullptr);
 // nrhs right-hand sides, column r at X + r * ldx (ldx a multiple of 4, >= n);
 // blocks up to 256 for more than 4 right-hand sides
 hipError_t launch_gesl_multi(const float* A, int64_t lda, int64_t n, int B, const float* inv,
-                             int64_t inv_stride, float* X, int64_t ldx, int nrhs, hipStream_t s);
+                             int64_t inv_stride, float* X, int64_t ldx, int nrhs, hipStream_t s,
+                             const float* binv = nullptr);
 
 // generation and residuals (gen.hip)
 hipError_t launch_generate(float* A, int64_t lda, int64_t n, int mode, uint64_t seed, double d,

@@ -420,7 +420,91 @@ __global__ __launch_bounds__(64) void trsm_reg_kernel(float* __restrict__ A, int
     }
 }
 
+// ----------------------------------------------------------------------------
+// Full inverses of the diagonal blocks for the substitution.  The strip
+// recurrences above, run on unit vectors, give for diagonal block k (with
+// L unit lower and U upper as the substitution uses them, M terms):
+//   left recurrence on e_r :  y = L^-1 e_r      -> OutL[r][i] = (L^-1)(i, r)
+//   top  recurrence on e_c :  y = -(U^T)^-1 e_c -> OutU[i][c] = -(U^-1)(c, i)
+// stored per block at out + k * 2 * B * B (OutL, then OutU), both row-major
+// B x B, so the substitution applies them as coalesced GEMVs.  Launch: grid
+// (B/32 vector groups, 2 types, blocks), one wave each; coefficients are read
+// straight from the factor (the transposed copy DT of the panel is gone).
+template <int B>
+__global__ __launch_bounds__(64) void block_inverse_kernel(const float* __restrict__ A,
+                                                           int64_t lda,
+                                                           const float* __restrict__ inv,
+                                                           int64_t inv_stride,
+                                                           float* __restrict__ out) {
+    constexpr int NQ = B / 32;
+    const int lane = threadIdx.x;
+    const int lr = lane & 31;
+    const int v0 = blockIdx.x * 32;
+    const bool top = blockIdx.y == 0;
+    const int64_t k = blockIdx.z;
+    const float* D = A + k * B * lda + k * B;
+    // coefficient X[p][i]: top C[i][p] = D[i][p], left C[i][p] = D[p][i]
+    const int64_t sp = top ? 1 : lda, si = top ? lda : 1;
+    const float* G = inv + k * inv_stride + (top ? 0 : NQ * 1024);
+    float* o = out + k * 2 * (int64_t)B * B + (top ? (int64_t)B * B : 0);
+
+    f32x16 Y[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        f32x16 acc;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[r] = (q * 32 + acc_row(r, lane) == v0 + lr) ? 1.0f : 0.0f;
+        float c[2][16];
+        if (q > 0) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r)
+                c[0][r] = D[(int64_t)acc_row(r, lane) * sp + (int64_t)(q * 32 + lr) * si];
+        }
+#pragma unroll
+        for (int pb = 0; pb < q; ++pb) {
+            if (pb + 1 < q) {
+#pragma unroll
+                for (int r = 0; r < 16; ++r)
+                    c[(pb + 1) & 1][r] = D[(int64_t)((pb + 1) * 32 + acc_row(r, lane)) * sp +
+                                           (int64_t)(q * 32 + lr) * si];
+            }
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc = mfma32(c[pb & 1][r], Y[pb][r], acc);
+        }
+        const float* Gq = G + q * 1024;
+        f32x16 y = {0};
+#pragma unroll
+        for (int r = 0; r < 16; ++r) y = mfma32(Gq[acc_row(r, lane) * 32 + lr], acc[r], y);
+        Y[q] = y;
+        // element i = q*32 + acc_row(r) of vector v0 + lr
+        if (top) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r)
+                o[(int64_t)(q * 32 + acc_row(r, lane)) * B + v0 + lr] = y[r];
+        } else {
+#pragma unroll
+            for (int r = 0; r < 16; ++r)
+                o[(int64_t)(v0 + lr) * B + q * 32 + acc_row(r, lane)] = y[r];
+        }
+    }
+}
+
 }  // namespace
+
+hipError_t launch_block_inverse(const float* A, int64_t lda, int64_t n, int B, const float* inv,
+                                int64_t inv_stride, float* out, hipStream_t s) {
+    if (n % B) return hipErrorInvalidValue;
+    const int NQ = B / 32;
+    dim3 grid(NQ, 2, (unsigned)(n / B));
+    switch (B) {
+        case 32: block_inverse_kernel<32><<<grid, 64, 0, s>>>(A, lda, inv, inv_stride, out); break;
+        case 64: block_inverse_kernel<64><<<grid, 64, 0, s>>>(A, lda, inv, inv_stride, out); break;
+        case 128: block_inverse_kernel<128><<<grid, 64, 0, s>>>(A, lda, inv, inv_stride, out); break;
+        case 256: block_inverse_kernel<256><<<grid, 64, 0, s>>>(A, lda, inv, inv_stride, out); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
 
 hipError_t launch_prep(const float* A, int64_t lda, int64_t off, int B, float* DT, float* inv,
                        hipStream_t s) {
