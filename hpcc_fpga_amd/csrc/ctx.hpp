@@ -92,6 +92,19 @@ struct hpl_ctx {
     size_t capRhsB = 0, capRhsX = 0;
     bool factored = false;
 
+    // partial pivoting (api_pivot.hip): hpl_gefa_dev factorises P M = L U when
+    // `pivoting` is set; the factor then holds L (every row exchange applied),
+    // U and -1/u_kk like the unpivoted encoding, and `pivoted` is set.
+    bool pivoting = false;
+    bool pivoted = false;
+    void* dPv = nullptr;        // one block: the arrays below and the panel scratch
+    size_t capPv = 0;
+    int32_t* dPiv = nullptr;    // ipvt in gefa_ref's meaning
+    int32_t* dPerm = nullptr;   // (P M)(i, :) = M(perm[i], :)
+    float* dDiagU = nullptr;    // u_kk
+    int pv_max_groups = 0;      // co-resident panel workgroups at block pv_B
+    uint32_t pv_B = 0;
+
     // roofline accounting of the trailing update
     bool prof = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
@@ -132,5 +145,18 @@ uint64_t device_lda(uint64_t n);
 // The full diagonal-block inverses of the resident factor (computed once per
 // factorisation, for blocks up to 256); dBInv stays null for larger blocks.
 hpl_status ensure_block_inverses(hpl_ctx* c);
+
+// Device buffer of at least `bytes` (grown with hipMalloc in the context's own
+// mode, an error in external mode).
+hpl_status ensure_mem(hpl_ctx* c, void** p, size_t* cap, size_t bytes);
+// Factorisation workspaces of the resident system at block B.
+hpl_status alloc_factor_workspace(hpl_ctx* c, uint32_t B);
+// Partial-pivoting factorisation of the resident matrix (FP32, B <= 256) on
+// stream s0, leaving the factor, dPiv, dPerm, dDiagU and the GESL inverses.
+hpl_status gefa_pivot(hpl_ctx* c, uint32_t B);
+// Rewrite a downloaded pivoted factor (host, row-major, lda) into gefa_ref's
+// layout: u_kk on the diagonal, each column's multipliers in the row order of
+// its own step.
+hpl_status pivot_factor_to_host(hpl_ctx* c, float* a, uint64_t lda);
 
 }  // namespace hplx

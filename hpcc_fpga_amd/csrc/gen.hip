@@ -24,13 +24,14 @@ __global__ void gen_offdiag_kernel(float* __restrict__ A, int64_t lda, int64_t n
             A[r * lda + c] = hpl_u24(key, (uint64_t)(r * n + c)) - 0.5f;
 }
 
-// mode 0: M(i,i) += n, b_i = sum_j M(i,j) (double, j increasing)
+// mode 0: M(i,i) += n, b_i = sum_j M(i,j) (double, j increasing);
+// mode 2: the same without the diagonal term (no diagonal dominance)
 __global__ void gen_mode0_kernel(float* __restrict__ A, int64_t lda, int64_t n,
                                  float* __restrict__ b, double* __restrict__ b64,
-                                 double* __restrict__ xt) {
+                                 double* __restrict__ xt, int dominant) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    A[i * lda + i] += (float)n;
+    if (dominant) A[i * lda + i] += (float)n;
     double s = 0.0;
     for (int64_t j = 0; j < n; ++j) s += (double)A[j * lda + i];
     if (b) b[i] = (float)s;
@@ -132,8 +133,8 @@ hipError_t launch_generate(float* A, int64_t lda, int64_t n, int mode, uint64_t 
     dim3 g1((unsigned)std::min<int64_t>((n + 255) / 256, 64), (unsigned)std::min<int64_t>(n, 8192));
     gen_offdiag_kernel<<<g1, 256, 0, s>>>(A, lda, n, key);
     const unsigned g = (unsigned)((n + 255) / 256);
-    if (mode == 0) {
-        gen_mode0_kernel<<<g, 256, 0, s>>>(A, lda, n, b, b64, xt);
+    if (mode == 0 || mode == 2) {
+        gen_mode0_kernel<<<g, 256, 0, s>>>(A, lda, n, b, b64, xt, mode == 0);
     } else {
         if (!xt) return hipErrorInvalidValue;
         gen_mode1_diag_kernel<<<g, 256, 0, s>>>(A, lda, n, d, hpl_stream_key(seed, 1), xt);
@@ -278,11 +279,12 @@ struct ColScales {
 
 // Xs[r][i] = (float)(R[r][i] * scale[r])
 __global__ void d2f_multi_kernel(const double* __restrict__ R, int64_t ldr, float* __restrict__ Xs,
-                                 int64_t ldxs, int64_t n, ColScales sc) {
+                                 int64_t ldxs, int64_t n, ColScales sc,
+                                 const int* __restrict__ perm) {
     const int r = blockIdx.y;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x)
-        Xs[r * ldxs + i] = (float)(R[r * ldr + i] * sc.v[r]);
+        Xs[r * ldxs + i] = (float)(R[r * ldr + (perm ? (int64_t)perm[i] : i)] * sc.v[r]);
 }
 
 // X[r][i] = scale[r] * D[r][i] (assign) or X[r][i] += scale[r] * D[r][i]
@@ -320,12 +322,12 @@ hipError_t launch_residual_multi(const float* A, int64_t lda, int64_t n, const d
 }
 
 hipError_t launch_d2f_multi(const double* R, int64_t ldr, float* Xs, int64_t ldxs, int64_t n,
-                            int nrhs, const double* scale, hipStream_t s) {
+                            int nrhs, const double* scale, hipStream_t s, const int* perm) {
     if (nrhs < 1 || nrhs > kMaxMultiRhs) return hipErrorInvalidValue;
     ColScales sc{};
     for (int r = 0; r < nrhs; ++r) sc.v[r] = scale[r];
     dim3 g((unsigned)std::min<int64_t>((n + 255) / 256, 1024), (unsigned)nrhs);
-    d2f_multi_kernel<<<g, 256, 0, s>>>(R, ldr, Xs, ldxs, n, sc);
+    d2f_multi_kernel<<<g, 256, 0, s>>>(R, ldr, Xs, ldxs, n, sc, perm);
     return hipGetLastError();
 }
 

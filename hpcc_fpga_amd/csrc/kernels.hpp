@@ -21,6 +21,10 @@ hipError_t launch_trsm(float* A, int64_t lda, int64_t off, int64_t m, int B, con
                        void* lpT = nullptr, void* lpL = nullptr, int64_t ldlp = 0,
                        float* scT = nullptr, float* scL = nullptr);
 
+// the top strip alone (rows off..off+B-1, columns off+B..off+B+m-1), FP32, B <= 256
+hipError_t launch_trsm_top(float* A, int64_t lda, int64_t off, int64_t m, int B, const float* DT,
+                           const float* inv, hipStream_t s);
+
 // trailing update (gemm.hip)
 hipError_t launch_gemm_update_f32(float* C, int64_t ldc, const float* Lt, int64_t ldl,
                                   const float* T, int64_t ldt, int M, int N, int K,
@@ -74,6 +78,35 @@ hipError_t launch_gesl_multi(const float* A, int64_t lda, int64_t n, int B, cons
                              int64_t inv_stride, float* X, int64_t ldx, int nrhs, hipStream_t s,
                              const float* binv = nullptr);
 
+// partial pivoting (pivot.hip); X is the transposed matrix X[i][j] = M(i, j)
+hipError_t launch_transpose_square(float* A, int64_t ld, int64_t n, hipStream_t s);
+// rows per workgroup and dynamic LDS bytes of the panel kernel at block B
+int panel_pivot_rows(int B);
+size_t panel_pivot_lds(int B);
+// allow the panel's LDS and return how many of its workgroups are co-resident
+hipError_t panel_pivot_prepare(int B, int* max_groups);
+// factorise panel k0 (m = n - k0 rows of X, B columns) with partial pivoting;
+// pub: 2 * G * (B+1) floats, krow: 2 * (B+1), keys: B, ctr: one counter
+// (G = ceil(m / panel_pivot_rows(B)) <= max_groups); writes the panel (-1/u on
+// the diagonal), Lt[c][pos] for pos >= k0 + B, diagU/ipvt at k0.., and the row
+// of origin src[pos] of positions k0..n-1.  status |= 1: zero or non-finite
+// pivot; |= 2: the grid barrier timed out.
+hipError_t launch_panel_pivot(float* X, int64_t ldx, int64_t n, int64_t k0, int B, float* pub,
+                              float* krow, unsigned long long* keys, unsigned* ctr, float* Lt,
+                              int64_t ldl, float* diagU, int* ipvt, int* src, int* status,
+                              int max_groups, hipStream_t s);
+// apply panel k0's row moves to the columns outside it and to perm; list:
+// 4 B ints, count: one int, Y: 2 B x ldy floats
+hipError_t launch_row_moves(float* X, int64_t ldx, int64_t n, int64_t k0, int B, const int* ipvt,
+                            const int* src, int* list, int* count, float* Y, int64_t ldy,
+                            int* perm, hipStream_t s);
+hipError_t launch_iota(int* p, int64_t n, hipStream_t s);
+// unit lower triangle of X's diagonal block at k0 into D (B x B)
+hipError_t launch_unit_lower(const float* X, int64_t ldx, int64_t k0, int B, float* D,
+                             hipStream_t s);
+// x[i] = b[perm[i]]
+hipError_t launch_gather_f32(float* x, const float* b, const int* perm, int64_t n, hipStream_t s);
+
 // generation and residuals (gen.hip)
 hipError_t launch_generate(float* A, int64_t lda, int64_t n, int mode, uint64_t seed, double d,
                            float* b, double* b64, double* xt, hipStream_t s);
@@ -98,9 +131,11 @@ hipError_t launch_residual_multi(const float* A, int64_t lda, int64_t n, const d
                                  int64_t ldx, const double* Bv, int64_t ldb, int nrhs,
                                  double* work, double* R, int64_t ldr, double* out,
                                  hipStream_t s);
-// Xs[r] = (float)(R[r] * scale[r]);  X[r] = scale[r] * D[r] (+ X[r])
+// Xs[r][i] = (float)(R[r][perm[i]] * scale[r]) (perm null: identity);
+// X[r] = scale[r] * D[r] (+ X[r])
 hipError_t launch_d2f_multi(const double* R, int64_t ldr, float* Xs, int64_t ldxs, int64_t n,
-                            int nrhs, const double* scale, hipStream_t s);
+                            int nrhs, const double* scale, hipStream_t s,
+                            const int* perm = nullptr);
 hipError_t launch_axpy_multi(double* X, int64_t ldx, const float* D, int64_t ldd, int64_t n,
                              int nrhs, const double* scale, int assign, hipStream_t s);
 

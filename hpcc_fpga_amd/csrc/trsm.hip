@@ -546,6 +546,28 @@ static hipError_t launch_trsm_t(float* A, int64_t lda, int64_t off, int64_t m, i
     return hipGetLastError();
 }
 
+hipError_t launch_trsm_top(float* A, int64_t lda, int64_t off, int64_t m, int B, const float* DT,
+                           const float* inv, hipStream_t s) {
+    if (m <= 0) return hipSuccess;
+    if (m % 32) return hipErrorInvalidValue;
+    const int groups = (int)(m / 32);
+#define HPL_TRSM_TOP(BB)                                                                      \
+    case BB:                                                                                  \
+        trsm_reg_kernel<BB, float><<<groups, 64, 0, s>>>(A, lda, off, m, DT, inv, nullptr, 0, \
+                                                         groups, nullptr, nullptr, 0, nullptr, \
+                                                         nullptr);                            \
+        break;
+    switch (B) {
+        HPL_TRSM_TOP(32)
+        HPL_TRSM_TOP(64)
+        HPL_TRSM_TOP(128)
+        HPL_TRSM_TOP(256)
+        default: return hipErrorInvalidValue;
+    }
+#undef HPL_TRSM_TOP
+    return hipGetLastError();
+}
+
 hipError_t launch_trsm(float* A, int64_t lda, int64_t off, int64_t m, int B, const float* DT,
                        const float* inv, float* Lt, int64_t ldt, hipStream_t s, int lp_prec,
                        void* lpT, void* lpL, int64_t ldlp, float* scT, float* scL) {

@@ -145,6 +145,10 @@ def load_library():
         "hpl_batch_solve": ([P, i32, i32, dbl, C.POINTER(_BatchStats)], i32),
         "hpl_batch_get": ([P, i32, P, C.POINTER(i32), C.POINTER(dbl)], i32),
         "hpl_batch_get_system": ([P, i32, P, P], i32),
+        "hpl_gefa_pvt": ([P, P, u64, u64, u32, P, T], i32),
+        "hpl_gesl_pvt": ([P, P, P, P, u64, u64, T], i32),
+        "hpl_set_pivoting": ([P, i32], i32),
+        "hpl_get_pivots": ([P, P], i32),
         "hpl_set_profiling": ([P, i32], i32),
         "hpl_kernel_stats": ([P, C.POINTER(u64), C.POINTER(dbl), C.POINTER(dbl)], i32),
     }
@@ -217,6 +221,36 @@ class Context:
         self._check(self._L.hpl_gesl(self._c, lu.ctypes.data, b.ctypes.data, lu.shape[0],
                                      lu.shape[1], C.byref(t)))
         return Timing(t.gefa_s, t.gesl_s, t.h2d_s, t.d2h_s)
+
+    # -- partial pivoting (gefa_ref / gesl_ref)
+    def gefa_pvt(self, a: np.ndarray, block: int):
+        """Factorise ``a`` in place with partial pivoting (gefa_ref's layout);
+        returns (ipvt, Timing)."""
+        a = _f32(a)
+        ipvt = np.empty(a.shape[0], np.int32)
+        t = _Timing()
+        self._check(self._L.hpl_gefa_pvt(self._c, a.ctypes.data, a.shape[0], a.shape[1], block,
+                                         ipvt.ctypes.data, C.byref(t)))
+        return ipvt, Timing(t.gefa_s, t.gesl_s, t.h2d_s, t.d2h_s)
+
+    def gesl_pvt(self, lu: np.ndarray, b: np.ndarray, ipvt: np.ndarray) -> Timing:
+        """Solve M x = b with a gefa_pvt factor; b is overwritten with x."""
+        lu, b = _f32(lu), _f32(b)
+        ipvt = np.ascontiguousarray(ipvt, np.int32)
+        t = _Timing()
+        self._check(self._L.hpl_gesl_pvt(self._c, lu.ctypes.data, b.ctypes.data,
+                                         ipvt.ctypes.data, lu.shape[0], lu.shape[1],
+                                         C.byref(t)))
+        return Timing(t.gefa_s, t.gesl_s, t.h2d_s, t.d2h_s)
+
+    def set_pivoting(self, enable: bool = True):
+        """Device-resident factorisations with (True) or without partial pivoting."""
+        self._check(self._L.hpl_set_pivoting(self._c, int(enable)))
+
+    def pivots(self) -> np.ndarray:
+        ipvt = np.empty(self.n, np.int32)
+        self._check(self._L.hpl_get_pivots(self._c, ipvt.ctypes.data))
+        return ipvt
 
     # -- device-resident path
     def keep_original(self, keep: bool = True):

@@ -94,6 +94,31 @@ hpl_status hpl_gefa(hpl_ctx* ctx, float* a, uint64_t n, uint64_t lda, uint32_t b
 hpl_status hpl_gesl(hpl_ctx* ctx, const float* lu, float* b, uint64_t n, uint64_t lda,
                     hpl_timing* timing);
 
+/* ---- partial pivoting (the reference's --uniform path) -------------------
+ * LU with partial pivoting by rows of M as gefa_ref / gesl_ref
+ * (LINPACK/src/host/linpack_data.cpp:72-156, chosen by --uniform,
+ * linpack_benchmark.hpp:69,440-447): column k takes the first row of largest
+ * |M(i,k)|, i >= k.  FP32 trailing update; block a power of two in [32, 256]
+ * dividing n.
+ * hpl_gefa_pvt replaces gefa_ref: it factorises a in place into gefa_ref's
+ * layout (the multipliers of column k in the row order of step k, u_kk on the
+ * diagonal) and writes ipvt (ipvt[k] = the row exchanged with row k at step
+ * k, ipvt[n-1] = n-1).  hpl_gesl_pvt replaces gesl_ref: it solves M x = b with
+ * such a factor and overwrites b with x.  A zero or non-finite pivot returns
+ * HPL_ERR_ZERO_PIVOT (gefa_ref divides by it). */
+hpl_status hpl_gefa_pvt(hpl_ctx* ctx, float* a, uint64_t n, uint64_t lda, uint32_t block,
+                        int32_t* ipvt, hpl_timing* timing);
+hpl_status hpl_gesl_pvt(hpl_ctx* ctx, const float* lu, float* b, const int32_t* ipvt, uint64_t n,
+                        uint64_t lda, hpl_timing* timing);
+/* With pivoting on (default off), hpl_gefa_dev factorises P M = L U (FP32
+ * trailing update only), hpl_gesl_dev / hpl_solve_ir / hpl_solve_gmres /
+ * hpl_solve_rhs64 solve with that factor, and hpl_download returns it in
+ * gefa_ref's layout.  No separate reference counterpart: it selects the
+ * gefa_ref path of the device-resident calls below. */
+hpl_status hpl_set_pivoting(hpl_ctx* ctx, int enable);
+/* ipvt of the resident pivoted factor (n entries, gefa_ref's meaning). */
+hpl_status hpl_get_pivots(hpl_ctx* ctx, int32_t* ipvt);
+
 /* ---- device-resident path -------------------------------------------------
  * The context holds one resident system (A, b, x).  Repetitions then time
  * only device work, as the reference times GEFA after the upload
@@ -106,7 +131,9 @@ hpl_status hpl_upload(hpl_ctx* ctx, const float* a, uint64_t n, uint64_t lda, co
  *  mode 0: M(i,j) = U[-0.5,0.5) off the diagonal, M(i,i) += n, b = M*1 (FP64
  *          sums rounded to FP32);
  *  mode 1: M(i,i) = d * sum_{j!=i} |M(i,j)|, x_true = U[-1,1), b = M x_true
- *          in FP64 (kept in FP64 for refinement, rounded copy in FP32).
+ *          in FP64 (kept in FP64 for refinement, rounded copy in FP32);
+ *  mode 2: M(i,j) = U[-0.5,0.5) everywhere (not diagonally dominant: the
+ *          pivoting workload), b = M*1 as in mode 0.
  * Element (r,c) of the row-major storage is a pure function of (seed, r*n+c),
  * so the result is bit-identical to the CPU generator of the same name. */
 hpl_status hpl_generate(hpl_ctx* ctx, uint64_t n, int mode, uint64_t seed, double d);

@@ -104,6 +104,8 @@ float orc_hash_uniform(uint64_t seed, uint64_t stream, uint64_t k) {
 
 // mode 0 ("add_n"): M(i,i) += n (float add), b = M*1, row sums of M taken in
 //                   double in increasing column order and rounded to float.
+// mode 2 ("uniform"): as mode 0 without the diagonal term: every entry is
+//                   U[-0.5,0.5), no diagonal dominance (the pivoting workload).
 // mode 1 ("scaled"): M(i,i) = d * sum_{j!=i} |M(i,j)| (double, rounded to
 //                   float), x_true_i = 2*u24(stream 1, i) - 1, b = M x_true
 //                   in double (b64) and rounded to float (b).
@@ -115,8 +117,9 @@ void orc_generate_sized(float* A, float* b, double* b64, double* xtrue,
     for (int64_t r = 0; r < n; r++)
         for (int64_t c = 0; c < n; c++)
             A[lda * r + c] = u24(key, (uint64_t)(r * n + c)) - 0.5f;
-    if (mode == 0) {
-        for (int64_t i = 0; i < n; i++) A[lda * i + i] += (float)n;
+    if (mode == 0 || mode == 2) {
+        if (mode == 0)
+            for (int64_t i = 0; i < n; i++) A[lda * i + i] += (float)n;
         for (int64_t i = 0; i < n; i++) {
             double s = 0.0;
             for (int64_t j = 0; j < n; j++) s += (double)A[lda * j + i];
@@ -213,6 +216,54 @@ void orc_gesl_nopvt(const float* a, float* b, int64_t n, int64_t lda) {
         float scale = t[k] * a[lda * k + k];
         t[k] = -scale;
         for (int64_t i = 0; i < k; i++) t[i] += scale * a[lda * k + i];
+    }
+    std::memcpy(b, t.data(), sizeof(float) * n);
+}
+
+// ---------------------------------------------------------------------------
+// gefa_ref, LINPACK/src/host/linpack_data.cpp:72-119: LU with partial pivoting
+// by rows of M (LINPACK dgefa).  Column k: the pivot is the first index of the
+// largest |M(i,k)|, i >= k (strict '<' keeps the earlier index on ties, :80-86);
+// rows k and p are exchanged in columns k..n-1 only (:88-92), so the
+// multipliers of earlier columns keep the row order of their own step; the
+// multipliers are M(i,k) * (-1.0 / M(k,k)) evaluated in double (the double
+// literal promotes, :96-98) and rounded to float; the update is a float
+// product added in float (:100-105).  The diagonal keeps u_kk.  ipvt[k] = p,
+// ipvt[n-1] = n-1.
+// ---------------------------------------------------------------------------
+void orc_gefa_pvt(float* a, int64_t n, int64_t lda, int32_t* ipvt) {
+    for (int64_t i = 0; i < n; i++) ipvt[i] = (int32_t)i;
+    for (int64_t k = 0; k < n - 1; k++) {
+        float max_val = std::fabs(a[k * lda + k]);
+        int64_t p = k;
+        for (int64_t i = k + 1; i < n; i++) {
+            if (max_val < std::fabs(a[k * lda + i])) {
+                p = i;
+                max_val = std::fabs(a[k * lda + i]);
+            }
+        }
+        for (int64_t i = k; i < n; i++) std::swap(a[i * lda + k], a[i * lda + p]);
+        ipvt[k] = (int32_t)p;
+        const double inv = -1.0 / (double)a[k * lda + k];
+        for (int64_t i = k + 1; i < n; i++) a[k * lda + i] = (float)((double)a[k * lda + i] * inv);
+        for (int64_t j = k + 1; j < n; j++)
+            for (int64_t i = k + 1; i < n; i++)
+                a[j * lda + i] += a[k * lda + i] * a[j * lda + k];
+    }
+}
+
+// gesl_ref, LINPACK/src/host/linpack_data.cpp:121-156: forward elimination
+// with the row exchanges of gefa_ref applied step by step, then back
+// substitution dividing by u_kk.
+void orc_gesl_pvt(const float* a, float* b, const int32_t* ipvt, int64_t n, int64_t lda) {
+    std::vector<float> t(b, b + n);
+    for (int64_t k = 0; k < n - 1; k++) {
+        if (ipvt[k] != k) std::swap(t[k], t[ipvt[k]]);
+        for (int64_t i = k + 1; i < n; i++) t[i] += t[k] * a[lda * k + i];
+    }
+    for (int64_t k = n - 1; k >= 0; k--) {
+        t[k] = t[k] / a[lda * k + k];
+        for (int64_t i = 0; i < k; i++) t[i] -= t[k] * a[lda * k + i];
     }
     std::memcpy(b, t.data(), sizeof(float) * n);
 }

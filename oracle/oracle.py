@@ -51,6 +51,9 @@ def lib():
         L.orc_gefa_nopvt_omp.argtypes = [_f32p, _i64, _i64, C.c_int]
         L.orc_gesl_nopvt.argtypes = [_f32p, _f32p, _i64, _i64]
         L.orc_gefa_nopvt_f64.argtypes = [_f64p, _i64, _i64]
+        _i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+        L.orc_gefa_pvt.argtypes = [_f32p, _i64, _i64, _i32p]
+        L.orc_gesl_pvt.argtypes = [_f32p, _f32p, _i32p, _i64, _i64]
         L.orc_gesl_nopvt_f64.argtypes = [_f64p, _f64p, _i64, _i64]
         for fn in (L.orc_top_update, L.orc_left_update):
             fn.argtypes = [C.c_void_p, _i64, C.c_void_p, _i64, _i64]
@@ -148,6 +151,56 @@ def gesl_nopvt(LU: np.ndarray, b: np.ndarray) -> np.ndarray:
     lu = np.ascontiguousarray(LU, np.float32)
     lib().orc_gesl_nopvt(lu, x, lu.shape[0], lu.shape[1])
     return x
+
+
+def gefa_pvt(A: np.ndarray):
+    """gefa_ref (partial pivoting, LINPACK/src/host/linpack_data.cpp:72-119).
+
+    Returns (F, ipvt): F in gefa_ref's layout (negated multipliers below M's
+    diagonal in the row order of their own step, u_kk on the diagonal)."""
+    a = np.ascontiguousarray(A, np.float32).copy()
+    ipvt = np.empty(a.shape[0], np.int32)
+    lib().orc_gefa_pvt(a, a.shape[0], a.shape[1], ipvt)
+    return a, ipvt
+
+
+def gesl_pvt(F: np.ndarray, b: np.ndarray, ipvt: np.ndarray) -> np.ndarray:
+    """gesl_ref (linpack_data.cpp:121-156) with a factor of gefa_pvt."""
+    x = np.ascontiguousarray(b, np.float32).copy()
+    f = np.ascontiguousarray(F, np.float32)
+    lib().orc_gesl_pvt(f, x, np.ascontiguousarray(ipvt, np.int32), f.shape[0], f.shape[1])
+    return x
+
+
+def pvt_to_lapack(F: np.ndarray, ipvt: np.ndarray) -> np.ndarray:
+    """The multipliers of a gefa_pvt factor with every later row exchange
+    applied (LAPACK getrf's convention: L of P M = L U).  Pure data movement:
+    exchange j of step j moves M rows j and ipvt[j] in the columns k < j."""
+    a = np.array(F, np.float32, copy=True)
+    for j, p in enumerate(np.asarray(ipvt)):
+        if p != j:
+            a[:j, [j, p]] = a[:j, [p, j]]
+    return a
+
+
+def lapack_to_pvt(F: np.ndarray, ipvt: np.ndarray) -> np.ndarray:
+    """Inverse of pvt_to_lapack (the exchanges undone in reverse order)."""
+    a = np.array(F, np.float32, copy=True)
+    piv = np.asarray(ipvt)
+    for j in range(len(piv) - 1, -1, -1):
+        p = piv[j]
+        if p != j:
+            a[:j, [j, p]] = a[:j, [p, j]]
+    return a
+
+
+def pivot_permutation(ipvt: np.ndarray) -> np.ndarray:
+    """perm with (P M)(i, :) = M(perm[i], :) for the exchanges of ipvt applied
+    in order."""
+    perm = np.arange(len(ipvt))
+    for j, p in enumerate(np.asarray(ipvt)):
+        perm[[j, p]] = perm[[p, j]]
+    return perm
 
 
 def gefa_nopvt_f64(A: np.ndarray) -> np.ndarray:

@@ -16,6 +16,13 @@ Each oracle function is checked against something other than itself:
   *_f64             - numpy.linalg.solve, the reference's FGMRES (gmres_ref);
   top/left/inner    - triangular-solve identities and a plain matmul;
   gefa_blocked      - bit-identical to gefa_nopvt;
+  gefa_pvt/gesl_pvt - LAPACK getrf (scipy): same pivots (first index on exact
+                      ties, as isamax), same L and U after the layout
+                      conversion, P M = L U; brute force 3x3 with a known
+                      pivot; the double-precision multiplier on an input where
+                      float evaluation differs; numpy.linalg.solve; the
+                      reference's own ReferenceSolveWithPivoting check
+                      (test_host_reference_implementations.cpp:82-89);
   residn_ref/norms  - direct numpy evaluation of the reference formulas.
 """
 import os
@@ -337,3 +344,107 @@ def test_lp_factor_error_ordered_by_precision():
     ref = O.gefa_blocked(A, 64)
     err = {p: np.max(np.abs(O.gefa_blocked_lp(A, 64, p) - ref)) for p in ("fp16", "bf16", "fp8")}
     assert 0 < err["fp16"] < err["bf16"] < err["fp8"]
+
+
+# --------------------------------------------------------------------------- partial pivoting
+def _lapack_lu_from_pvt(F, ipvt):
+    """(L, U) of P M = L U from a gefa_pvt factor (negated multipliers)."""
+    G = O.pvt_to_lapack(F, ipvt).T.astype(np.float64)  # M layout
+    L = np.eye(G.shape[0]) - np.tril(G, -1)
+    U = np.triu(G)
+    return L, U
+
+
+@pytest.mark.parametrize("kind", ["uniform01", "normal"])
+def test_gefa_pvt_against_lapack_getrf(kind):
+    n = 96
+    rng = np.random.default_rng(7)
+    A = (rng.random((n, n)) if kind == "uniform01" else rng.standard_normal((n, n))).astype(
+        np.float32)
+    F, ipvt = O.gefa_pvt(A)
+    M = A.T.astype(np.float64)
+    lu, piv = sla.lu_factor(M)
+    assert np.array_equal(ipvt, piv)
+    L, U = _lapack_lu_from_pvt(F, ipvt)
+    assert np.allclose(L, np.tril(lu, -1) + np.eye(n), atol=2e-5)
+    assert np.allclose(U, np.triu(lu), atol=2e-5 * np.abs(lu).max())
+    perm = O.pivot_permutation(ipvt)
+    assert np.allclose(L @ U, M[perm], atol=1e-5 * np.abs(M).max())
+    assert np.array_equal(O.lapack_to_pvt(O.pvt_to_lapack(F, ipvt), ipvt), F)
+
+
+def test_gefa_pvt_ties_take_first_index():
+    # small integers give exact ties in |M(i,k)| at many steps; LAPACK's isamax
+    # and gefa_ref's strict '<' both keep the first
+    rng = np.random.default_rng(5)
+    n = 40
+    A = rng.integers(-2, 3, size=(n, n)).astype(np.float32)
+    A[np.arange(n), np.arange(n)] += 0.5
+    F, ipvt = O.gefa_pvt(A)
+    _, piv = sla.lu_factor(A.T.astype(np.float64))
+    assert np.array_equal(ipvt, piv)
+    # column 0 of M is A[0]: the pivot is the first of its largest magnitudes
+    col = np.abs(A[0].astype(np.float64))
+    assert ipvt[0] == int(np.flatnonzero(col == col.max())[0])
+
+
+def test_gefa_pvt_brute_force_3x3():
+    # M rows: [1 2 3], [4 5 6], [7 8 10]; column 0 pivots on row 2 (|7|)
+    M = np.array([[1, 2, 3], [4, 5, 6], [7, 8, 10]], np.float64)
+    A = np.ascontiguousarray(M.T.astype(np.float32))
+    F, ipvt = O.gefa_pvt(A)
+    assert ipvt[0] == 2
+    # by hand: after exchanging rows 0 and 2, l = (4/7, 1/7); the 2x2 remainder
+    # [[5-32/7, 6-40/7], [2-8/7, 3-10/7]] = [[3/7, 2/7], [6/7, 11/7]] pivots on 6/7
+    assert ipvt[1] == 2 and ipvt[2] == 2
+    L, U = _lapack_lu_from_pvt(F, ipvt)
+    P = M[O.pivot_permutation(ipvt)]
+    assert np.allclose(L @ U, P, atol=1e-5)
+    assert U[0, 0] == 7.0 and U[1, 1] == pytest.approx(6 / 7) and F[0, 0] == 7.0
+    # gefa_ref's layout: step 0 stored -(4/7, 1/7) in rows 1, 2 before step 1
+    # exchanged rows 1 and 2 in columns >= 1 only
+    assert F[0, 1] == pytest.approx(-4 / 7) and F[0, 2] == pytest.approx(-1 / 7)
+
+
+def test_gefa_pvt_multiplier_in_double():
+    # gefa_ref: a[k*lda+i] *= -1.0 / a[k*lda+k] -- the double literal makes
+    # the reciprocal and the product double, rounded to float once.  Find an
+    # input where evaluating it in float would round differently.
+    rng = np.random.default_rng(0)
+    for _ in range(100000):
+        akk, aik = rng.random(2).astype(np.float32) + np.float32(0.5)
+        if aik >= akk:
+            continue
+        dbl = np.float32(np.float64(aik) * (-1.0 / np.float64(akk)))
+        flt = np.float32(aik * (np.float32(-1.0) / akk))
+        if dbl != flt:
+            break
+    else:
+        pytest.fail("no distinguishing input found")
+    A = np.array([[akk, aik], [0.25, 1.0]], np.float32)  # M = [[akk, .25], [aik, 1]]
+    F, ipvt = O.gefa_pvt(A)
+    assert ipvt[0] == 0
+    assert F[0, 1] == dbl and F[0, 1] != flt
+
+
+def test_gesl_pvt_against_numpy_and_reference_check():
+    n = 128
+    A, b, _, normb = O.generate_ref(n, seed=0, diag_dominant=False)
+    F, ipvt = O.gefa_pvt(A)
+    x = O.gesl_pvt(F, b, ipvt)
+    xs = np.linalg.solve(A.T.astype(np.float64), b.astype(np.float64))
+    assert np.max(np.abs(x - xs)) < 1e-3
+    _, residn = O.residn_ref(x, normb)  # ReferenceSolveWithPivoting: validateOutput
+    assert residn < 1.0
+    # a zero leading pivot: the unpivoted factorisation divides by zero, the
+    # pivoted one exchanges it away
+    A0 = A.copy()
+    A0[0, 0] = 0.0
+    b0 = A0.T.astype(np.float64).sum(axis=1).astype(np.float32)
+    with np.errstate(all="ignore"):
+        xn = O.gesl_nopvt(O.gefa_nopvt(A0), b0)
+    assert not np.all(np.isfinite(xn))
+    F0, ipvt0 = O.gefa_pvt(A0)
+    assert ipvt0[0] != 0
+    assert O.hpl_scaled_residual(A0, O.gesl_pvt(F0, b0, ipvt0).astype(np.float64),
+                                 b0.astype(np.float64)) < 1.0
