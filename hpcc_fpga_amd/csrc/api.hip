@@ -350,6 +350,7 @@ hpl_status check_status(hpl_ctx* c) {
     if (h & 2)
         return fail(c, HPL_ERR_HIP,
                     "pivot panel: a grid barrier timed out (workgroups not co-resident)");
+    if (h & 4) return fail(c, HPL_ERR_HIP, "pivot panel: inconsistent pivot key");
     if (h) return fail(c, HPL_ERR_ZERO_PIVOT, "zero or non-finite pivot met during GEFA");
     return HPL_SUCCESS;
 }

@@ -37,7 +37,7 @@ using hplx::fail;
 
 // Byte offsets inside the context's pivot block.
 struct PvLayout {
-    size_t piv, perm, src, diag, pub, krow, keys, ctr, list, y, d, total;
+    size_t piv, perm, src, diag, pub, krow, keys, sync, list, y, d, total;
 };
 
 size_t al256(size_t x) { return (x + 255) / 256 * 256; }
@@ -57,7 +57,7 @@ PvLayout pv_layout(uint64_t n, uint32_t B, int64_t groups) {
     L.pub = take(2 * (size_t)groups * (B + 1) * sizeof(float));
     L.krow = take(2 * (size_t)(B + 1) * sizeof(float));
     L.keys = take((size_t)B * sizeof(unsigned long long));
-    L.ctr = take(sizeof(unsigned));
+    L.sync = take(sizeof(unsigned));
     L.list = take((4 * (size_t)B + 1) * sizeof(int));
     L.y = take(2 * (size_t)B * n * sizeof(float));
     L.d = take((size_t)B * B * sizeof(float));
@@ -121,7 +121,7 @@ hpl_status gefa_pivot(hpl_ctx* c, uint32_t B) {
     float* pub = reinterpret_cast<float*>(base + L.pub);
     float* krow = reinterpret_cast<float*>(base + L.krow);
     auto* keys = reinterpret_cast<unsigned long long*>(base + L.keys);
-    auto* ctr = reinterpret_cast<unsigned*>(base + L.ctr);
+    auto* sync = reinterpret_cast<unsigned*>(base + L.sync);
     int* list = reinterpret_cast<int*>(base + L.list);
     int* count = list + 4 * B;
     float* Y = reinterpret_cast<float*>(base + L.y);
@@ -133,7 +133,7 @@ hpl_status gefa_pivot(hpl_ctx* c, uint32_t B) {
     PCHK(c, launch_iota(c->dPerm, n, s));
     for (int64_t k = 0; k < N; ++k) {
         const int64_t k0 = k * B, m1 = n - k0 - B;
-        PCHK(c, launch_panel_pivot(c->dA, lda, n, k0, (int)B, pub, krow, keys, ctr, c->dLt, n,
+        PCHK(c, launch_panel_pivot(c->dA, lda, n, k0, (int)B, pub, krow, keys, sync, c->dLt, n,
                                    c->dDiagU, c->dPiv, src, c->dStatus, c->pv_max_groups, s));
         PCHK(c, launch_row_moves(c->dA, lda, n, k0, (int)B, c->dPiv, src, list, count, Y, n,
                                  c->dPerm, s));

@@ -80,17 +80,20 @@ hipError_t launch_gesl_multi(const float* A, int64_t lda, int64_t n, int B, cons
 
 // partial pivoting (pivot.hip); X is the transposed matrix X[i][j] = M(i, j)
 hipError_t launch_transpose_square(float* A, int64_t ld, int64_t n, hipStream_t s);
-// rows per workgroup and dynamic LDS bytes of the panel kernel at block B
+// rows per workgroup and dynamic LDS bytes of the panel kernel at block B;
+// workgroups of a panel of m rows
 int panel_pivot_rows(int B);
 size_t panel_pivot_lds(int B);
+int panel_pivot_groups(int64_t m, int B);
 // allow the panel's LDS and return how many of its workgroups are co-resident
 hipError_t panel_pivot_prepare(int B, int* max_groups);
 // factorise panel k0 (m = n - k0 rows of X, B columns) with partial pivoting;
 // pub: 2 * G * (B+1) floats, krow: 2 * (B+1), keys: B, ctr: one counter
-// (G = ceil(m / panel_pivot_rows(B)) <= max_groups); writes the panel (-1/u on
+// (G = panel_pivot_groups(m, B) <= max_groups); writes the panel (-1/u on
 // the diagonal), Lt[c][pos] for pos >= k0 + B, diagU/ipvt at k0.., and the row
 // of origin src[pos] of positions k0..n-1.  status |= 1: zero or non-finite
-// pivot; |= 2: the grid barrier timed out.
+// pivot; |= 2: the grid barrier timed out; |= 4: a pivot key outside the
+// panel was read (a broken barrier; the row is left in place).
 hipError_t launch_panel_pivot(float* X, int64_t ldx, int64_t n, int64_t k0, int B, float* pub,
                               float* krow, unsigned long long* keys, unsigned* ctr, float* Lt,
                               int64_t ldl, float* diagU, int* ipvt, int* src, int* status,
