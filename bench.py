@@ -95,6 +95,39 @@ def mixed_precision_section(ctx, n, B):
     return out
 
 
+def pivoted_section(ctx, n, B, reps=3):
+    """Partial pivoting (gefa_ref's path, --uniform): M = U[-0.5,0.5) with no
+    diagonal dominance, b = M 1; FP32 GEFA+GESL, then FP64 refinement with the
+    pivoted FP32 factor."""
+    import numpy as np
+
+    flops = 2.0 / 3.0 * n ** 3 + 2.0 * n * n
+    ctx.keep_original(True)
+    ctx.generate(n, 2, 42)
+    ctx.set_pivoting(True)
+    tot, lu = [], []
+    for _ in range(reps + 1):
+        ctx.restore()
+        a = ctx.gefa_dev(B)
+        b = ctx.gesl_dev()
+        tot.append(a.gefa_s + b.gesl_s)
+        lu.append(a.gefa_s)
+    tot, lu = tot[1:], lu[1:]  # the first run warms up this shape
+    rn, an, xn, _ = ctx.residual()
+    eps = float(np.finfo(np.float32).eps)
+    ir = ctx.solve_ir(B, "fp32", max_iter=10, check=False)
+    ctx.set_pivoting(False)
+    t = float(np.median(tot))
+    print(f"[bench] pivoted n={n}: {flops / t / 1e9:.1f} GFLOP/s", file=sys.stderr, flush=True)
+    return {"workload": f"partial pivoting, n={n}, block {B}, U[-0.5,0.5) (not diagonally "
+                        f"dominant), b = A*1", "gflops": round(flops / t / 1e9, 1),
+            "median_ms": round(t * 1e3, 3), "gefa_ms_median": round(float(np.median(lu)) * 1e3, 3),
+            "hpl_scaled_residual": rn / (eps * an * xn * n),
+            "refined": {"iterations": ir["iterations"], "backward_error": ir["backward_error"],
+                        "converged": bool(ir["converged"]),
+                        "total_ms": round(ir["total_s"] * 1e3, 3)}}
+
+
 def batched_section(ctx, count=4096, seed=7):
     """4096 systems, n log-uniform in [32, 1024], FP16 trailing update + FP64
     refinement, one HIP-graph replay per solve; workgroup sizes autotuned."""
@@ -266,7 +299,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=5120)
     ap.add_argument("--no-extra", action="store_true",
-                    help="skip the mixed-precision and batched sections")
+                    help="skip the pivoted, mixed-precision, batched and serving sections")
     ap.add_argument("--mixed-n", type=int, default=65536)
     ap.add_argument("--serve-rate", type=float, default=None,
                     help="Poisson arrival rate (req/s); default 70%% of measured capacity")
@@ -328,6 +361,7 @@ def main():
 
     extra = {}
     if not args.no_extra:
+        extra["pivoted"] = pivoted_section(ctx, n, 128)
         extra["mixed_precision"] = mixed_precision_section(ctx, args.mixed_n, B)
         extra["batched"] = batched_section(ctx)
     ctx.close()

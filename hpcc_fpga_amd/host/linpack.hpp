@@ -131,14 +131,18 @@ namespace hip {
 // The HIP counterpart of execution::pcie::calculate: upload once (outside the
 // timers, as execution_pcie.hpp:123-127), then per repetition restore the
 // matrix on the device and time GEFA and GESL with HIP events.  Leaves the
-// factor in data.A and the solution x in data.b.
+// factor in data.A and the solution x in data.b.  A --uniform matrix is
+// factorised with partial pivoting (gefa_ref, linpack_data.cpp:72-119): data.A
+// then holds gefa_ref's layout and data.ipvt its pivots.
 inline std::map<std::string, std::vector<double>> calculate(const Settings& config,
                                                              LinpackData& data) {
     hpl_ctx* c = config.context->get();
     const uint64_t n = config.programSettings->matrixSize;
     const uint32_t B = config.programSettings->blockSize;
+    const bool pivoting = !config.programSettings->isDiagonallyDominant;
     hpl_timing t{};
     ASSERT_HPL(c, hpl_keep_original(c, 1));
+    ASSERT_HPL(c, hpl_set_pivoting(c, pivoting ? 1 : 0));
     ASSERT_HPL(c, hpl_upload(c, data.A, n, n, data.b, &t));
     std::vector<double> gefa, gesl;
     for (unsigned rep = 0; rep < config.programSettings->numRepetitions; ++rep) {
@@ -149,6 +153,7 @@ inline std::map<std::string, std::vector<double>> calculate(const Settings& conf
         gesl.push_back(t.gesl_s);
     }
     ASSERT_HPL(c, hpl_download(c, data.A, n, data.b, &t));
+    if (pivoting) ASSERT_HPL(c, hpl_get_pivots(c, data.ipvt));
     return {{"gefa", gefa}, {"gesl", gesl}};
 }
 
@@ -191,9 +196,9 @@ class LinpackBenchmark
     bool checkInputParameters() override {
         auto& s = *executionSettings->programSettings;
         if (s.communicationType == CommunicationType::hip) {
-            if (!s.isDiagonallyDominant) {
-                std::cerr << "ERROR: the HIP backend factorises without pivoting; --uniform "
-                             "matrices are not supported"
+            if (!s.isDiagonallyDominant && s.blockSize > 256) {
+                std::cerr << "ERROR: partial pivoting (--uniform) needs a block size in "
+                             "[32, 256] (b in 5..8)"
                           << std::endl;
                 return false;
             }
@@ -217,7 +222,7 @@ class LinpackBenchmark
         if (s.generator == "ref")
             generateReference(*d, n, s.isDiagonallyDominant);
         else
-            generateSized(*d, n, s.seed);
+            generateSized(*d, n, s.seed, s.isDiagonallyDominant);
         for (size_t i = 0; i < n; ++i) d->ipvt[i] = (int)i;
         return d;
     }
@@ -255,15 +260,16 @@ class LinpackBenchmark
     }
 
     // Counter-based generator of the sized workloads (the device generator
-    // hpl_generate mode 0 produces the same bits): A = U[-0.5,0.5) on a 2^-24
-    // grid from splitmix64 of (seed, r*n+c), A_ii += n, b = row sums of M in
-    // double rounded to float.
+    // hpl_generate mode 0, or mode 2 without dominance, produces the same
+    // bits): A = U[-0.5,0.5) on a 2^-24 grid from splitmix64 of (seed, r*n+c),
+    // A_ii += n when diagonally dominant, b = row sums of M in double rounded
+    // to float.
     static uint64_t mix64(uint64_t z) {
         z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
         z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
         return z ^ (z >> 31);
     }
-    void generateSized(LinpackData& d, size_t n, uint64_t seed) {
+    void generateSized(LinpackData& d, size_t n, uint64_t seed, bool dd) {
         const uint64_t key = mix64(seed ^ (0ULL << 56));
         float* A = d.A;
 #pragma omp parallel for schedule(static)
@@ -272,7 +278,8 @@ class LinpackBenchmark
                 const uint64_t z = mix64(key + ((uint64_t)r * n + c + 1) * 0x9E3779B97F4A7C15ULL);
                 A[r * n + c] = (float)(uint32_t)(z >> 40) * (1.0f / 16777216.0f) - 0.5f;
             }
-        for (size_t i = 0; i < n; ++i) A[i * n + i] += (float)n;
+        if (dd)
+            for (size_t i = 0; i < n; ++i) A[i * n + i] += (float)n;
         std::vector<double> s(n, 0.0);
         const int nt = omp_get_max_threads();
         const size_t chunk = (n + nt - 1) / nt;
@@ -287,7 +294,13 @@ class LinpackBenchmark
         for (size_t i = 0; i < n; ++i) {
             d.b[i] = (float)s[i];
             if (d.b[i] > d.normb) d.normb = d.b[i];
-            if (A[i * n + i] > d.norma) d.norma = A[i * n + i];
+            if (dd && A[i * n + i] > d.norma) d.norma = A[i * n + i];
+        }
+        if (!dd) {  // the running maximum of the generated values, as the reference keeps it
+            float m = 0.0f;
+#pragma omp parallel for reduction(max : m)
+            for (long long e = 0; e < (long long)(n * n); ++e) m = std::max(m, A[e]);
+            d.norma = m;
         }
     }
 

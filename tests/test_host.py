@@ -96,3 +96,27 @@ def test_full_run_and_json(tmp_path, gen):
     assert d["errors"]["hpl_scaled_residual"] < 1
     assert d["settings"]["Matrix Size"] == 1024
     assert d["settings"]["FPGA Torus"] == {"P": 1, "Q": 1}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("gen", ["ref", "sized"])
+def test_uniform_runs_with_pivoting(tmp_path, gen):
+    """--uniform: partial pivoting on the device (gefa_ref's path), validated by
+    the reference's residn check (test_host_reference_implementations.cpp:82-89)."""
+    out = tmp_path / "u.json"
+    print(f"host --uniform generator={gen}", flush=True)
+    r = run("-f", "HIP", "-m", "8", "-b", "7", "-n", "2", "--uniform", "--generator", gen,
+            "--dump-json", str(out))
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "Validation: SUCCESS!" in r.stdout
+    assert re.search(r"Diagonally Dominant\s+No", r.stdout)
+    d = json.loads(out.read_text())
+    assert d["validated"] is True
+    assert d["errors"]["residual_norm"] < 1
+    assert d["errors"]["hpl_scaled_residual"] < 1
+
+
+def test_uniform_rejects_large_blocks():
+    r = run("-f", "HIP", "-m", "2", "-b", "9", "--uniform", "--test")
+    assert r.returncode == 1
+    assert "partial pivoting (--uniform) needs a block size in [32, 256]" in r.stderr + r.stdout
