@@ -25,11 +25,14 @@ struct hpl_ctx {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     hipEvent_t evA = nullptr, evP = nullptr;  // next-panel region updated / panel ready
     bool lookahead = true;
-    // Bulk trailing update every `depth` steps (K = depth * B).  The FP32
-    // update is MFMA-bound, so 1; the 16-bit update is bound by the FP32 C
-    // traffic (8 bytes per element per pass), which `depth` divides.
+    // Bulk trailing update every `depth` steps (K = depth * B).  FP32: 3 --
+    // at n = 32768, B = 256 the bulk launches run at 0.75 of the FP32 MFMA
+    // peak instead of 0.69-0.70 at depth 1 (fewer, longer launches that the
+    // look-ahead panel disturbs less) for about 0.6% more GEFA time; deeper
+    // is slower.  The 16-bit and FP8 updates are bound by the FP32 C traffic
+    // (8 bytes per element per pass), which `depth` divides: 8.
     int depth = 1;
-    int depth_fp32 = 1, depth_lp = 8;
+    int depth_fp32 = 3, depth_lp = 8;
     std::string err;
 
     // resident system
