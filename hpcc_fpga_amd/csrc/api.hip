@@ -564,7 +564,9 @@ void hpl_destroy(hpl_ctx* c) {
                         c->dLpT, c->dLpL, (void*)c->dScT, (void*)c->dScL, (void*)c->dX64,
                         (void*)c->dIrR, (void*)c->dIrXs, (void*)c->dIrOut, (void*)c->dRhsB,
                         (void*)c->dRhsX, (void*)c->dGmV, (void*)c->dGmW, (void*)c->dGmC,
-                        (void*)c->dGmWork, (void*)c->dBInv, c->dPv})
+                        (void*)c->dGmWork, (void*)c->dBInv, c->dPv, (void*)c->dA64,
+                        (void*)c->dA064, (void*)c->dB64s, (void*)c->dX64s, (void*)c->dLt64,
+                        (void*)c->dWork64, (void*)c->dOut64})
             if (p) (void)hipFree(p);
     for (auto& e : c->ev) {
         (void)hipEventDestroy(e.first);

@@ -1,0 +1,266 @@
+// FP64 LINPACK behind the C ABI: the reference built with DATA_TYPE=double
+// (its "_DP" build; gefa_ref_nopvt / gesl_ref_nopvt with HOST_DATA_TYPE
+// double, LINPACK/src/host/linpack_data.cpp:166-229).  Kernels in f64.hip.
+//
+// Schedule (stream s0): blocked right-looking LU with block 64 and a delayed
+// bulk update of depth kF64Depth, as the FP32 schedule: at step k the next
+// panel's row and column blocks receive every pending update, the next panel
+// is factorised (diagonal LU, strip solves), and the rest of the trailing
+// matrix is updated every kF64Depth steps with K = kF64Depth * 64 -- K = 64
+// alone would leave the FP64 update at 8 flop per byte of C traffic, under
+// the device's ~10.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "../../include/hpl_hip.h"
+#include "ctx.hpp"
+#include "kernels.hpp"
+
+namespace {
+
+using hplx::fail;
+
+constexpr int kF64Depth = 2;
+
+#define FCHK(ctx, expr)                                                                   \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return fail((ctx), e_ == hipErrorOutOfMemory ? HPL_ERR_OUT_OF_MEMORY : HPL_ERR_HIP, \
+                        "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, \
+                        __LINE__);                                                        \
+    } while (0)
+
+template <typename T>
+hpl_status grow(hpl_ctx* c, T*& p, size_t& cap, size_t bytes) {
+    void* v = p;
+    hpl_status st = hplx::ensure_mem(c, &v, &cap, bytes);
+    p = static_cast<T*>(v);
+    return st;
+}
+
+hpl_status alloc_f64(hpl_ctx* c, uint64_t n) {
+    if (n == 0 || n % (uint64_t)f64_block())
+        return fail(c, HPL_ERR_INVALID_ARG, "FP64 path: n=%llu must be a positive multiple of %d",
+                    (unsigned long long)n, f64_block());
+    const uint64_t lda = hplx::device_lda(n);
+    hpl_status st = grow(c, c->dA64, c->capA64, n * lda * sizeof(double));
+    if (!st) st = grow(c, c->dA064, c->capA064, n * lda * sizeof(double));
+    if (!st && n * sizeof(double) > c->capV64) {
+        size_t cap = 0;
+        st = grow(c, c->dB64s, cap, n * sizeof(double));
+        cap = 0;
+        if (!st) st = grow(c, c->dX64s, cap, n * sizeof(double));
+        cap = 0;
+        if (!st) st = grow(c, c->dOut64, cap, 4 * sizeof(double));
+        if (!st) c->capV64 = n * sizeof(double);
+    }
+    if (!st)
+        st = grow(c, c->dLt64, c->capLt64,
+                  2 * (size_t)kF64Depth * f64_block() * n * sizeof(double));
+    if (!st) st = grow(c, c->dWork64, c->capWork64, residual_f64_work((int64_t)n) * sizeof(double));
+    if (!st) st = hplx::ensure_mem(c, reinterpret_cast<void**>(&c->dStatus), &c->capStatus, 64);
+    if (st) return st;
+    c->n64 = n;
+    c->lda64 = lda;
+    c->have_orig64 = false;
+    c->factored64 = false;
+    return HPL_SUCCESS;
+}
+
+int64_t lt_slot64(int64_t k) {
+    return ((k / kF64Depth) % 2) * kF64Depth + (k % kF64Depth);
+}
+
+// C += L T over rows [r0, r0+M) and columns [c0, c0+N), steps k0 .. k0+K/64-1
+hpl_status update64(hpl_ctx* c, int64_t k0, int64_t r0, int64_t c0, int64_t M, int64_t N, int K) {
+    if (M <= 0 || N <= 0) return HPL_SUCCESS;
+    const int64_t n = (int64_t)c->n64, lda = (int64_t)c->lda64, B = f64_block();
+    const double* Lt = c->dLt64 + lt_slot64(k0) * B * n + r0;
+    const double* T = c->dA64 + k0 * B * lda + c0;
+    FCHK(c, launch_update_f64(c->dA64 + r0 * lda + c0, lda, Lt, n, T, lda, (int)M, (int)N, K,
+                              c->s0));
+    return HPL_SUCCESS;
+}
+
+hpl_status panel64(hpl_ctx* c, int64_t k) {
+    const int64_t n = (int64_t)c->n64, lda = (int64_t)c->lda64, B = f64_block();
+    const int64_t off = k * B;
+    FCHK(c, launch_lu_f64(c->dA64, lda, off, c->dStatus, c->s0));
+    const int64_t m = n - off - B;
+    if (m > 0)
+        FCHK(c, launch_strips_f64(c->dA64, lda, off, m, c->dLt64 + lt_slot64(k) * B * n + off + B,
+                                  n, c->s0));
+    return HPL_SUCCESS;
+}
+
+hpl_status schedule64(hpl_ctx* c) {
+    const int64_t n = (int64_t)c->n64, B = f64_block(), N = n / B;
+    hpl_status r = panel64(c, 0);
+    if (r) return r;
+    for (int64_t k = 0; k + 1 < N; ++k) {
+        const int64_t k1 = (k + 1) * B, m = n - k1;
+        const int64_t j = k % kF64Depth;
+        const int K = (int)((j + 1) * B);
+        const int64_t k0 = k - j;
+        if (m <= B) {
+            r = update64(c, k0, k1, k1, m, m, K);
+            if (r) return r;
+            r = panel64(c, k + 1);
+            if (r) return r;
+            continue;
+        }
+        r = update64(c, k0, k1, k1, B, m, K);  // row block of the next panel
+        if (!r) r = update64(c, k0, k1 + B, k1, m - B, B, K);  // column block
+        if (!r) r = panel64(c, k + 1);
+        if (!r && j == kF64Depth - 1) r = update64(c, k0, k1 + B, k1 + B, m - B, m - B, K);
+        if (r) return r;
+    }
+    return HPL_SUCCESS;
+}
+
+double seconds(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, a, b);
+    return ms * 1e-3;
+}
+
+}  // namespace
+
+extern "C" {
+
+hpl_status hpl_upload_f64(hpl_ctx* c, const double* a, uint64_t n, uint64_t lda, const double* b,
+                          hpl_timing* t) {
+    if (!c) return HPL_ERR_INVALID_ARG;
+    if (!a || lda < n) return fail(c, HPL_ERR_INVALID_ARG, "bad upload_f64 arguments");
+    FCHK(c, hipSetDevice(c->device));
+    hpl_status st = alloc_f64(c, n);
+    if (st) return st;
+    FCHK(c, hipEventRecord(c->e0, c->s0));
+    FCHK(c, hipMemcpy2DAsync(c->dA64, c->lda64 * sizeof(double), a, lda * sizeof(double),
+                             n * sizeof(double), n, hipMemcpyHostToDevice, c->s0));
+    if (b)
+        FCHK(c, hipMemcpyAsync(c->dB64s, b, n * sizeof(double), hipMemcpyHostToDevice, c->s0));
+    else
+        FCHK(c, launch_fill_f64(c->dB64s, (int64_t)n, 0.0, c->s0));
+    FCHK(c, hipMemcpyAsync(c->dA064, c->dA64, n * c->lda64 * sizeof(double),
+                           hipMemcpyDeviceToDevice, c->s0));
+    FCHK(c, hipEventRecord(c->e1, c->s0));
+    FCHK(c, hipStreamSynchronize(c->s0));
+    c->have_orig64 = true;
+    if (t) t->h2d_s = seconds(c->e0, c->e1);
+    return HPL_SUCCESS;
+}
+
+hpl_status hpl_generate_f64(hpl_ctx* c, uint64_t n, int mode, uint64_t seed, double d) {
+    if (!c) return HPL_ERR_INVALID_ARG;
+    // the FP32 generator's system, widened: M exactly as hpl_generate makes
+    // it, b = M x_true in FP64 from those values
+    hpl_status st = hpl_generate(c, n, mode, seed, d);
+    if (st) return st;
+    st = alloc_f64(c, n);
+    if (st) return st;
+    FCHK(c, launch_f32_to_f64(c->dA, (int64_t)c->lda, c->dA64, (int64_t)c->lda64, (int64_t)n,
+                              c->s0));
+    FCHK(c, hipMemcpyAsync(c->dB64s, c->db64, n * sizeof(double), hipMemcpyDeviceToDevice, c->s0));
+    FCHK(c, hipMemcpyAsync(c->dA064, c->dA64, n * c->lda64 * sizeof(double),
+                           hipMemcpyDeviceToDevice, c->s0));
+    FCHK(c, hipStreamSynchronize(c->s0));
+    c->have_orig64 = true;
+    return HPL_SUCCESS;
+}
+
+hpl_status hpl_restore_f64(hpl_ctx* c) {
+    if (!c) return HPL_ERR_INVALID_ARG;
+    if (!c->have_orig64) return fail(c, HPL_ERR_STATE, "no FP64 system resident");
+    FCHK(c, hipMemcpyAsync(c->dA64, c->dA064, c->n64 * c->lda64 * sizeof(double),
+                           hipMemcpyDeviceToDevice, c->s0));
+    FCHK(c, hipStreamSynchronize(c->s0));
+    c->factored64 = false;
+    return HPL_SUCCESS;
+}
+
+hpl_status hpl_gefa_dev_f64(hpl_ctx* c, hpl_timing* t) {
+    if (!c) return HPL_ERR_INVALID_ARG;
+    if (!c->dA64 || c->n64 == 0) return fail(c, HPL_ERR_STATE, "no FP64 system resident");
+    FCHK(c, hipSetDevice(c->device));
+    FCHK(c, hipMemsetAsync(c->dStatus, 0, sizeof(int), c->s0));
+    c->factored64 = false;
+    FCHK(c, hipEventRecord(c->e0, c->s0));
+    hpl_status st = schedule64(c);
+    if (st) return st;
+    FCHK(c, hipEventRecord(c->e1, c->s0));
+    FCHK(c, hipStreamSynchronize(c->s0));
+    if (t) t->gefa_s = seconds(c->e0, c->e1);
+    int h = 0;
+    FCHK(c, hipMemcpy(&h, c->dStatus, sizeof(int), hipMemcpyDeviceToHost));
+    if (h) return fail(c, HPL_ERR_ZERO_PIVOT, "zero or non-finite pivot met during FP64 GEFA");
+    c->factored64 = true;
+    return HPL_SUCCESS;
+}
+
+hpl_status hpl_gesl_dev_f64(hpl_ctx* c, hpl_timing* t) {
+    if (!c) return HPL_ERR_INVALID_ARG;
+    if (!c->factored64) return fail(c, HPL_ERR_STATE, "hpl_gesl_dev_f64 before a successful GEFA");
+    FCHK(c, hipSetDevice(c->device));
+    FCHK(c, hipEventRecord(c->e0, c->s0));
+    FCHK(c, hipMemcpyAsync(c->dX64s, c->dB64s, c->n64 * sizeof(double), hipMemcpyDeviceToDevice,
+                           c->s0));
+    FCHK(c, launch_gesl_f64(c->dA64, (int64_t)c->lda64, (int64_t)c->n64, c->dX64s, c->s0));
+    FCHK(c, hipEventRecord(c->e1, c->s0));
+    FCHK(c, hipStreamSynchronize(c->s0));
+    if (t) t->gesl_s = seconds(c->e0, c->e1);
+    return HPL_SUCCESS;
+}
+
+hpl_status hpl_download_f64(hpl_ctx* c, double* a, uint64_t lda, double* x, hpl_timing* t) {
+    if (!c) return HPL_ERR_INVALID_ARG;
+    if (!c->dA64) return fail(c, HPL_ERR_STATE, "no FP64 system resident");
+    if (a && lda < c->n64) return fail(c, HPL_ERR_INVALID_ARG, "lda < n");
+    FCHK(c, hipSetDevice(c->device));
+    FCHK(c, hipEventRecord(c->e0, c->s0));
+    if (a)
+        FCHK(c, hipMemcpy2DAsync(a, lda * sizeof(double), c->dA64, c->lda64 * sizeof(double),
+                                 c->n64 * sizeof(double), c->n64, hipMemcpyDeviceToHost, c->s0));
+    if (x)
+        FCHK(c, hipMemcpyAsync(x, c->dX64s, c->n64 * sizeof(double), hipMemcpyDeviceToHost, c->s0));
+    FCHK(c, hipEventRecord(c->e1, c->s0));
+    FCHK(c, hipStreamSynchronize(c->s0));
+    if (t) t->d2h_s = seconds(c->e0, c->e1);
+    return HPL_SUCCESS;
+}
+
+hpl_status hpl_residual_f64(hpl_ctx* c, double out[4]) {
+    if (!c || !out) return HPL_ERR_INVALID_ARG;
+    if (!c->have_orig64) return fail(c, HPL_ERR_STATE, "no FP64 system resident");
+    FCHK(c, hipSetDevice(c->device));
+    FCHK(c, launch_residual_f64(c->dA064, (int64_t)c->lda64, (int64_t)c->n64, c->dX64s, c->dB64s,
+                                c->dWork64, c->dOut64, c->s0));
+    FCHK(c, hipMemcpyAsync(out, c->dOut64, 4 * sizeof(double), hipMemcpyDeviceToHost, c->s0));
+    FCHK(c, hipStreamSynchronize(c->s0));
+    return HPL_SUCCESS;
+}
+
+hpl_status hpl_gefa_f64(hpl_ctx* c, double* a, uint64_t n, uint64_t lda, hpl_timing* t) {
+    if (!c) return HPL_ERR_INVALID_ARG;
+    hpl_status st = hpl_upload_f64(c, a, n, lda, nullptr, t);
+    if (!st) st = hpl_gefa_dev_f64(c, t);
+    if (!st) st = hpl_download_f64(c, a, lda, nullptr, t);
+    return st;
+}
+
+hpl_status hpl_gesl_f64(hpl_ctx* c, const double* lu, double* b, uint64_t n, uint64_t lda,
+                        hpl_timing* t) {
+    if (!c) return HPL_ERR_INVALID_ARG;
+    if (!lu || !b) return fail(c, HPL_ERR_INVALID_ARG, "bad gesl_f64 arguments");
+    hpl_status st = hpl_upload_f64(c, lu, n, lda, b, t);
+    if (st) return st;
+    c->factored64 = true;
+    st = hpl_gesl_dev_f64(c, t);
+    if (st) return st;
+    FCHK(c, hipMemcpy(b, c->dX64s, n * sizeof(double), hipMemcpyDeviceToHost));
+    return HPL_SUCCESS;
+}
+
+}  // extern "C"

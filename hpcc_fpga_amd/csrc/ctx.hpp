@@ -108,6 +108,20 @@ struct hpl_ctx {
     int pv_max_groups = 0;      // co-resident panel workgroups at block pv_B
     uint32_t pv_B = 0;
 
+    // FP64 system (api_f64.hip): matrix, kept original, b, x, transposed
+    // left strips (2 * kF64Depth slots of 64 x n) and the residual partials
+    double* dA64 = nullptr;
+    double* dA064 = nullptr;
+    double* dB64s = nullptr;
+    double* dX64s = nullptr;
+    double* dLt64 = nullptr;
+    double* dWork64 = nullptr;
+    double* dOut64 = nullptr;
+    size_t capA64 = 0, capA064 = 0, capV64 = 0, capLt64 = 0, capWork64 = 0;
+    uint64_t n64 = 0, lda64 = 0;
+    bool have_orig64 = false;
+    bool factored64 = false;
+
     // roofline accounting of the trailing update
     bool prof = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;

@@ -110,6 +110,24 @@ hipError_t launch_unit_lower(const float* X, int64_t ldx, int64_t k0, int B, flo
 // x[i] = b[perm[i]]
 hipError_t launch_gather_f32(float* x, const float* b, const int* perm, int64_t n, hipStream_t s);
 
+// FP64 path (f64.hip): block f64_block() = 64
+int f64_block();
+hipError_t launch_lu_f64(double* A, int64_t lda, int64_t off, int* status, hipStream_t s);
+// top strip (rows off.., columns off+64..off+64+m-1) and left strip (rows
+// off+64.., columns off..off+63); the left strip also into Lt[k][v] (ldl)
+hipError_t launch_strips_f64(double* A, int64_t lda, int64_t off, int64_t m, double* Lt,
+                             int64_t ldl, hipStream_t s);
+// C[r][c] += sum_k Lt[k][r] T[k][c] (K a multiple of 16, M and N even)
+hipError_t launch_update_f64(double* C, int64_t ldc, const double* Lt, int64_t ldl,
+                             const double* T, int64_t ldt, int M, int N, int K, hipStream_t s);
+// x := M^-1 x with the factor (gesl_ref_nopvt's order; n a multiple of 64)
+hipError_t launch_gesl_f64(const double* A, int64_t lda, int64_t n, double* x, hipStream_t s);
+int64_t residual_f64_work(int64_t n);
+hipError_t launch_residual_f64(const double* A, int64_t lda, int64_t n, const double* x,
+                               const double* b, double* work, double* out4, hipStream_t s);
+hipError_t launch_f32_to_f64(const float* src, int64_t lds, double* dst, int64_t ldd, int64_t n,
+                             hipStream_t s);
+
 // generation and residuals (gen.hip)
 hipError_t launch_generate(float* A, int64_t lda, int64_t n, int mode, uint64_t seed, double d,
                            float* b, double* b64, double* xt, hipStream_t s);

@@ -149,6 +149,15 @@ def load_library():
         "hpl_gesl_pvt": ([P, P, P, P, u64, u64, T], i32),
         "hpl_set_pivoting": ([P, i32], i32),
         "hpl_get_pivots": ([P, P], i32),
+        "hpl_gefa_f64": ([P, P, u64, u64, T], i32),
+        "hpl_gesl_f64": ([P, P, P, u64, u64, T], i32),
+        "hpl_upload_f64": ([P, P, u64, u64, P, T], i32),
+        "hpl_generate_f64": ([P, u64, i32, u64, dbl], i32),
+        "hpl_gefa_dev_f64": ([P, T], i32),
+        "hpl_gesl_dev_f64": ([P, T], i32),
+        "hpl_restore_f64": ([P], i32),
+        "hpl_download_f64": ([P, P, u64, P, T], i32),
+        "hpl_residual_f64": ([P, C.POINTER(C.c_double)], i32),
         "hpl_set_profiling": ([P, i32], i32),
         "hpl_kernel_stats": ([P, C.POINTER(u64), C.POINTER(dbl), C.POINTER(dbl)], i32),
     }
@@ -158,6 +167,12 @@ def load_library():
         fn.restype = res
     _lib = L
     return L
+
+
+def _f64(a: np.ndarray) -> np.ndarray:
+    if a.dtype != np.float64 or not a.flags["C_CONTIGUOUS"]:
+        raise TypeError("expected a C-contiguous float64 array")
+    return a
 
 
 def _f32(a: np.ndarray) -> np.ndarray:
@@ -176,6 +191,7 @@ class Context:
         if st != HPL_SUCCESS:
             raise HplError(st, self._L.hpl_status_string(st).decode())
         self.n = 0
+        self.n64 = 0
 
     # -- plumbing
     def _check(self, st: int):
@@ -251,6 +267,62 @@ class Context:
         ipvt = np.empty(self.n, np.int32)
         self._check(self._L.hpl_get_pivots(self._c, ipvt.ctypes.data))
         return ipvt
+
+    # -- FP64 (the reference's DATA_TYPE=double build)
+    def gefa_f64(self, a: np.ndarray) -> Timing:
+        """Factorise ``a`` (n x n row-major float64) in place."""
+        a = _f64(a)
+        t = _Timing()
+        self._check(self._L.hpl_gefa_f64(self._c, a.ctypes.data, a.shape[0], a.shape[1],
+                                         C.byref(t)))
+        return Timing(t.gefa_s, t.gesl_s, t.h2d_s, t.d2h_s)
+
+    def gesl_f64(self, lu: np.ndarray, b: np.ndarray) -> Timing:
+        lu, b = _f64(lu), _f64(b)
+        t = _Timing()
+        self._check(self._L.hpl_gesl_f64(self._c, lu.ctypes.data, b.ctypes.data, lu.shape[0],
+                                         lu.shape[1], C.byref(t)))
+        return Timing(t.gefa_s, t.gesl_s, t.h2d_s, t.d2h_s)
+
+    def upload_f64(self, a: np.ndarray, b: np.ndarray | None = None) -> Timing:
+        a = _f64(a)
+        t = _Timing()
+        bp = _f64(b).ctypes.data if b is not None else None
+        self._check(self._L.hpl_upload_f64(self._c, a.ctypes.data, a.shape[0], a.shape[1], bp,
+                                           C.byref(t)))
+        self.n64 = a.shape[0]
+        return Timing(t.gefa_s, t.gesl_s, t.h2d_s, t.d2h_s)
+
+    def generate_f64(self, n: int, mode: int = 0, seed: int = 42, d: float = 1.0):
+        self._check(self._L.hpl_generate_f64(self._c, n, mode, seed, d))
+        self.n, self.n64 = n, n
+
+    def gefa_dev_f64(self) -> Timing:
+        t = _Timing()
+        self._check(self._L.hpl_gefa_dev_f64(self._c, C.byref(t)))
+        return Timing(t.gefa_s, t.gesl_s, t.h2d_s, t.d2h_s)
+
+    def gesl_dev_f64(self) -> Timing:
+        t = _Timing()
+        self._check(self._L.hpl_gesl_dev_f64(self._c, C.byref(t)))
+        return Timing(t.gefa_s, t.gesl_s, t.h2d_s, t.d2h_s)
+
+    def restore_f64(self):
+        self._check(self._L.hpl_restore_f64(self._c))
+
+    def download_f64(self, matrix: bool = True, solution: bool = True):
+        n = self.n64
+        a = np.empty((n, n), np.float64) if matrix else None
+        x = np.empty(n, np.float64) if solution else None
+        t = _Timing()
+        self._check(self._L.hpl_download_f64(self._c, a.ctypes.data if matrix else None, n,
+                                             x.ctypes.data if solution else None, C.byref(t)))
+        return a, x
+
+    def residual_f64(self):
+        out = (C.c_double * 4)()
+        self._check(self._L.hpl_residual_f64(self._c, out))
+        return tuple(out)
 
     # -- device-resident path
     def keep_original(self, keep: bool = True):

@@ -119,6 +119,30 @@ hpl_status hpl_set_pivoting(hpl_ctx* ctx, int enable);
 /* ipvt of the resident pivoted factor (n entries, gefa_ref's meaning). */
 hpl_status hpl_get_pivots(hpl_ctx* ctx, int32_t* ipvt);
 
+/* ---- FP64: the reference built with DATA_TYPE=double ----------------------
+ * The "_DP" build of the reference runs the same LINPACK in double
+ * (HOST_DATA_TYPE double: gefa_ref_nopvt / gesl_ref_nopvt,
+ * LINPACK/src/host/linpack_data.cpp:166-229; CI builds it, .gitlab-ci.yml:220-228).
+ * Same storage and encoding as the FP32 calls; blocks of 64 (n must be a
+ * multiple of 64), FP64 MFMA trailing update.  hpl_gefa_f64 / hpl_gesl_f64
+ * are the host-buffer calls, as hpl_gefa / hpl_gesl; the device-resident
+ * calls keep one FP64 system (with an untouched copy for restore and the
+ * residual) beside the FP32 one. */
+hpl_status hpl_gefa_f64(hpl_ctx* ctx, double* a, uint64_t n, uint64_t lda, hpl_timing* timing);
+hpl_status hpl_gesl_f64(hpl_ctx* ctx, const double* lu, double* b, uint64_t n, uint64_t lda,
+                        hpl_timing* timing);
+hpl_status hpl_upload_f64(hpl_ctx* ctx, const double* a, uint64_t n, uint64_t lda,
+                          const double* b, hpl_timing* timing);
+/* The system of hpl_generate(n, mode, seed, d) widened to FP64 (b = M x_true
+ * in FP64 from the same values). */
+hpl_status hpl_generate_f64(hpl_ctx* ctx, uint64_t n, int mode, uint64_t seed, double d);
+hpl_status hpl_gefa_dev_f64(hpl_ctx* ctx, hpl_timing* timing);
+hpl_status hpl_gesl_dev_f64(hpl_ctx* ctx, hpl_timing* timing);
+hpl_status hpl_restore_f64(hpl_ctx* ctx);
+hpl_status hpl_download_f64(hpl_ctx* ctx, double* a, uint64_t lda, double* x, hpl_timing* timing);
+/* out = (||b - M x||inf, ||M||inf, ||x||inf, ||b||inf) against the untouched copy. */
+hpl_status hpl_residual_f64(hpl_ctx* ctx, double out[4]);
+
 /* ---- device-resident path -------------------------------------------------
  * The context holds one resident system (A, b, x).  Repetitions then time
  * only device work, as the reference times GEFA after the upload

@@ -74,6 +74,30 @@ void orc_generate_ref(float* A, float* b, int64_t n, int64_t lda,
     if (normb_out) *normb_out = normb;
 }
 
+// The same generator as the reference builds it with DATA_TYPE=double
+// (HOST_DATA_TYPE double, linpack_benchmark.hpp:222-295): the U(0,1) draws
+// are kept in double, the row and column sums are double sums.
+void orc_generate_ref_f64(double* A, double* b, int64_t n, int64_t lda, uint32_t seed,
+                          int diag_dominant) {
+    std::mt19937 gen(seed);
+    std::uniform_real_distribution<> dis(0.0, 1.0);
+    for (int64_t j = 0; j < n; j++)
+        for (int64_t i = 0; i < n; i++) A[lda * j + i] = dis(gen);
+    if (diag_dominant) {
+        for (int64_t r = 0; r < n; r++) {
+            A[lda * r + r] = 0.0;
+            double row_sum = 0.0;
+            for (int64_t i = 0; i < n; i++) row_sum += A[lda * r + i];
+            A[lda * r + r] = row_sum;
+        }
+    }
+    for (int64_t j = 0; j < n; j++) {
+        double col_sum = 0.0;
+        for (int64_t i = 0; i < n; i++) col_sum += A[lda * i + j];
+        b[j] = col_sum;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Counter-based generator for the sized workloads (no reference counterpart:
 // the reference has only the mt19937 generator above).  Element k of stream s

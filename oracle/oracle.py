@@ -51,6 +51,7 @@ def lib():
         L.orc_gefa_nopvt_omp.argtypes = [_f32p, _i64, _i64, C.c_int]
         L.orc_gesl_nopvt.argtypes = [_f32p, _f32p, _i64, _i64]
         L.orc_gefa_nopvt_f64.argtypes = [_f64p, _i64, _i64]
+        L.orc_generate_ref_f64.argtypes = [_f64p, _f64p, _i64, _i64, C.c_uint32, C.c_int]
         _i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
         L.orc_gefa_pvt.argtypes = [_f32p, _i64, _i64, _i32p]
         L.orc_gesl_pvt.argtypes = [_f32p, _f32p, _i32p, _i64, _i64]
@@ -98,6 +99,14 @@ def generate_ref(n: int, seed: int = 0, diag_dominant: bool = True):
     na, nb = C.c_float(), C.c_float()
     lib().orc_generate_ref(A, b, n, n, seed, int(diag_dominant), C.byref(na), C.byref(nb))
     return A, b, float(na.value), float(nb.value)
+
+
+def generate_ref_f64(n: int, seed: int = 0, diag_dominant: bool = True):
+    """Reference generator of the DATA_TYPE=double build. Returns (A, b)."""
+    A = np.empty((n, n), np.float64)
+    b = np.empty(n, np.float64)
+    lib().orc_generate_ref_f64(A, b, n, n, seed, int(diag_dominant))
+    return A, b
 
 
 def generate_sized(n: int, seed: int = 42, mode: int = 0, d: float = 1.0):

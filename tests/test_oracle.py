@@ -448,3 +448,17 @@ def test_gesl_pvt_against_numpy_and_reference_check():
     assert ipvt0[0] != 0
     assert O.hpl_scaled_residual(A0, O.gesl_pvt(F0, b0, ipvt0).astype(np.float64),
                                  b0.astype(np.float64)) < 1.0
+
+
+def test_ref_generator_f64_matches_float_draws():
+    """The DATA_TYPE=double generator draws the same mt19937 sequence; the
+    float build rounds each draw, the double build keeps it."""
+    n = 48
+    A64, b64 = O.generate_ref_f64(n, seed=3)
+    A32, b32, _, _ = O.generate_ref(n, seed=3)
+    off = ~np.eye(n, dtype=bool)
+    assert np.array_equal(A64.astype(np.float32)[off], A32[off])
+    assert np.allclose(np.diag(A64), A64.sum(axis=1) - np.diag(A64), rtol=1e-14)
+    assert np.allclose(b64, A64.sum(axis=0), rtol=1e-14)
+    x = np.linalg.solve(A64.T, b64)
+    assert np.max(np.abs(x - 1.0)) < 1e-13
