@@ -137,13 +137,19 @@ __device__ __forceinline__ void tile_coords64(int bid, int nblk, int tiles_m, in
     tn = in_g / gsize;
 }
 
-__global__ __launch_bounds__(UNT) void update_f64_kernel(double* __restrict__ C, int64_t ldc,
-                                                         const double* __restrict__ At,
-                                                         int64_t lda, const double* __restrict__ Bt,
-                                                         int64_t ldb, int M, int N, int K) {
+// Slabs are staged with global_load_lds: each wave instruction moves one
+// 1 KiB row of a slab (16 B per lane) to a wave-uniform LDS row; the padded
+// LDS row keeps the four k-rows of an operand read in distinct bank halves.
+// The accumulators start from C, so the epilogue is a plain store, and the
+// register budget allows two workgroups per compute unit.
+__global__ __launch_bounds__(UNT, 2) void update_f64_kernel(double* __restrict__ C, int64_t ldc,
+                                                            const double* __restrict__ At,
+                                                            int64_t lda,
+                                                            const double* __restrict__ Bt,
+                                                            int64_t ldb, int M, int N, int K) {
     constexpr int FM = 4, FN = 4;  // 16 x 16 blocks per 64 x 64 wave tile
     constexpr int LDS_ROW = UBM + UPAD;
-    constexpr int SLAB2 = UBK * UBM / 2 / UNT;  // double2 per thread per operand slab (4)
+    constexpr int ROWS_PER_WAVE = UBK / (UNT / 64);  // slab rows each wave moves (4)
     __shared__ double As[2][UBK][LDS_ROW];
     __shared__ double Bs[2][UBK][LDS_ROW];
     const int tiles_m = (M + UBM - 1) / UBM, tiles_n = (N + UBN - 1) / UBN;
@@ -154,28 +160,20 @@ __global__ __launch_bounds__(UNT) void update_f64_kernel(double* __restrict__ C,
     const int wm = wave & 1, wn = wave >> 1;
     const int l16 = lane & 15, lq = lane >> 4;
     const bool full = (m0 + UBM <= M) && (n0 + UBN <= N);
+    // lane l moves doubles 2l, 2l+1 of a row; clamped so edge tiles read in
+    // bounds (the clamped columns only reach rows/columns of C never stored)
+    const int ca = min(2 * lane, M - m0 - 2), cb = min(2 * lane, N - n0 - 2);
 
-    double2 ra[SLAB2], rb[SLAB2];
-    auto load_slab = [&](int k0) {
+    auto dma_slab = [&](int k0, int buf) {
 #pragma unroll
-        for (int s = 0; s < SLAB2; ++s) {
-            const int f = tid + UNT * s;  // double2 index in the slab
-            const int kk = f / (UBM / 2), cc = (f % (UBM / 2)) * 2;
-            ra[s] = (full || m0 + cc < M)
-                        ? *reinterpret_cast<const double2*>(At + (int64_t)(k0 + kk) * lda + m0 + cc)
-                        : double2{0.0, 0.0};
-            rb[s] = (full || n0 + cc < N)
-                        ? *reinterpret_cast<const double2*>(Bt + (int64_t)(k0 + kk) * ldb + n0 + cc)
-                        : double2{0.0, 0.0};
-        }
-    };
-    auto store_slab = [&](int buf) {
-#pragma unroll
-        for (int s = 0; s < SLAB2; ++s) {
-            const int f = tid + UNT * s;
-            const int kk = f / (UBM / 2), cc = (f % (UBM / 2)) * 2;
-            *reinterpret_cast<double2*>(&As[buf][kk][cc]) = ra[s];
-            *reinterpret_cast<double2*>(&Bs[buf][kk][cc]) = rb[s];
+        for (int s = 0; s < ROWS_PER_WAVE; ++s) {
+            const int row = wave * ROWS_PER_WAVE + s;
+            __builtin_amdgcn_global_load_lds(
+                At + (int64_t)(k0 + row) * lda + m0 + ca,
+                (__attribute__((address_space(3))) void*)(&As[buf][row][0]), 16, 0, 0);
+            __builtin_amdgcn_global_load_lds(
+                Bt + (int64_t)(k0 + row) * ldb + n0 + cb,
+                (__attribute__((address_space(3))) void*)(&Bs[buf][row][0]), 16, 0, 0);
         }
     };
     const int rbase = m0 + wm * 64, cbase = n0 + wn * 64;
@@ -183,16 +181,22 @@ __global__ __launch_bounds__(UNT) void update_f64_kernel(double* __restrict__ C,
 #pragma unroll
     for (int i = 0; i < FM; ++i)
 #pragma unroll
-        for (int j = 0; j < FN; ++j) acc[i][j] = f64x4{0.0, 0.0, 0.0, 0.0};
+        for (int j = 0; j < FN; ++j)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int row = rbase + i * 16 + lq + 4 * q;
+                const int col = cbase + j * 16 + l16;
+                acc[i][j][q] = (full || (row < M && col < N)) ? C[(int64_t)row * ldc + col] : 0.0;
+            }
 
-    load_slab(0);
-    store_slab(0);
+    dma_slab(0, 0);
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     const int nk = K / UBK;
     for (int kc = 0; kc < nk; ++kc) {
         const int buf = kc & 1;
         const bool more = kc + 1 < nk;
-        if (more) load_slab((kc + 1) * UBK);
+        if (more) dma_slab((kc + 1) * UBK, buf ^ 1);
 #pragma unroll
         for (int kq = 0; kq < UBK / 4; ++kq) {
             double a[FM], b[FN];
@@ -207,7 +211,7 @@ __global__ __launch_bounds__(UNT) void update_f64_kernel(double* __restrict__ C,
                     acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
         }
         if (more) {
-            store_slab(buf ^ 1);
+            __builtin_amdgcn_s_waitcnt(0);
             __syncthreads();
         }
     }
@@ -219,10 +223,7 @@ __global__ __launch_bounds__(UNT) void update_f64_kernel(double* __restrict__ C,
             for (int q = 0; q < 4; ++q) {
                 const int row = rbase + i * 16 + lq + 4 * q;
                 const int col = cbase + j * 16 + l16;
-                if (full || (row < M && col < N)) {
-                    double* p = C + (int64_t)row * ldc + col;
-                    *p = *p + acc[i][j][q];
-                }
+                if (full || (row < M && col < N)) C[(int64_t)row * ldc + col] = acc[i][j][q];
             }
 }
 
