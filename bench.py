@@ -128,6 +128,33 @@ def pivoted_section(ctx, n, B, reps=3):
                         "total_ms": round(ir["total_s"] * 1e3, 3)}}
 
 
+def fp64_section(ctx, n, reps=3):
+    """The reference built with DATA_TYPE=double: the same generator's system
+    in FP64, FP64 GEFA (FP64 MFMA update) + GESL, HPL residual in FP64."""
+    import numpy as np
+
+    flops = 2.0 / 3.0 * n ** 3 + 2.0 * n * n
+    ctx.generate_f64(n, 0, 42)
+    tot, lu = [], []
+    for r in range(reps + 1):
+        if r:
+            ctx.restore_f64()
+        a = ctx.gefa_dev_f64()
+        b = ctx.gesl_dev_f64()
+        tot.append(a.gefa_s + b.gesl_s)
+        lu.append(a.gefa_s)
+    tot, lu = tot[1:], lu[1:]
+    rn, an, xn, _ = ctx.residual_f64()
+    eps = float(np.finfo(np.float64).eps)
+    t = float(np.median(tot))
+    print(f"[bench] fp64 n={n}: {flops / t / 1e9:.1f} GFLOP/s", file=sys.stderr, flush=True)
+    return {"workload": f"FP64 (DATA_TYPE=double build), n={n}, block 64, U[-0.5,0.5) + n on "
+                        f"the diagonal, b = A*1", "gflops": round(flops / t / 1e9, 1),
+            "peak_fp64_tflops": 78.6, "median_ms": round(t * 1e3, 3),
+            "gefa_ms_median": round(float(np.median(lu)) * 1e3, 3),
+            "hpl_scaled_residual_fp64": rn / (eps * an * xn * n)}
+
+
 def batched_section(ctx, count=4096, seed=7):
     """4096 systems, n log-uniform in [32, 1024], FP16 trailing update + FP64
     refinement, one HIP-graph replay per solve; workgroup sizes autotuned."""
@@ -299,7 +326,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=5120)
     ap.add_argument("--no-extra", action="store_true",
-                    help="skip the pivoted, mixed-precision, batched and serving sections")
+                    help="skip the pivoted, FP64, mixed-precision, batched and serving sections")
     ap.add_argument("--mixed-n", type=int, default=65536)
     ap.add_argument("--serve-rate", type=float, default=None,
                     help="Poisson arrival rate (req/s); default 70%% of measured capacity")
@@ -362,6 +389,7 @@ def main():
     extra = {}
     if not args.no_extra:
         extra["pivoted"] = pivoted_section(ctx, n, 128)
+        extra["fp64"] = fp64_section(ctx, n)
         extra["mixed_precision"] = mixed_precision_section(ctx, args.mixed_n, B)
         extra["batched"] = batched_section(ctx)
     ctx.close()

@@ -3,15 +3,15 @@
 // double, LINPACK/src/host/linpack_data.cpp:166-229).  Kernels in f64.hip.
 //
 // Schedule (stream s0): blocked right-looking LU with block 64 and a delayed
-// bulk update of depth kF64Depth, as the FP32 schedule: at step k the next
-// panel's row and column blocks receive every pending update, the next panel
-// is factorised (diagonal LU, strip solves), and the rest of the trailing
-// matrix is updated every kF64Depth steps with K = kF64Depth * 64 -- K = 64
-// alone would leave the FP64 update at 8 flop per byte of C traffic, under
-// the device's ~10.
+// bulk update, as the FP32 schedule: at step k the next panel's row and
+// column blocks receive every pending update, the next panel is factorised
+// (diagonal LU, strip solves), and the rest of the trailing matrix is updated
+// every `depth` steps with K = depth * 64 -- K = 64 alone would leave the FP64
+// update at 8 flop per byte of C traffic, under the device's ~10.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 
 #include "../../include/hpl_hip.h"
 #include "ctx.hpp"
@@ -21,7 +21,18 @@ namespace {
 
 using hplx::fail;
 
-constexpr int kF64Depth = 2;
+// Update depth: the bulk update runs every depth steps with K = depth * 64.
+// Measured at n = 32768: depth 1/2/3/4/6/8 -> 27.3/35.0/39.1/41.3/43.5/44.4
+// TFLOP/s (n = 16384 peaks at 4-6); 6 by default, HPL_F64_DEPTH (1..8)
+// overrides it.
+constexpr int kF64DepthMax = 8;
+int f64_depth() {
+    static const int d = [] {
+        const char* e = getenv("HPL_F64_DEPTH");
+        return e ? std::max(1, std::min(kF64DepthMax, atoi(e))) : 6;
+    }();
+    return d;
+}
 
 #define FCHK(ctx, expr)                                                                   \
     do {                                                                                  \
@@ -58,7 +69,7 @@ hpl_status alloc_f64(hpl_ctx* c, uint64_t n) {
     }
     if (!st)
         st = grow(c, c->dLt64, c->capLt64,
-                  2 * (size_t)kF64Depth * f64_block() * n * sizeof(double));
+                  2 * (size_t)kF64DepthMax * f64_block() * n * sizeof(double));
     if (!st) st = grow(c, c->dWork64, c->capWork64, residual_f64_work((int64_t)n) * sizeof(double));
     if (!st) st = hplx::ensure_mem(c, reinterpret_cast<void**>(&c->dStatus), &c->capStatus, 64);
     if (st) return st;
@@ -70,7 +81,8 @@ hpl_status alloc_f64(hpl_ctx* c, uint64_t n) {
 }
 
 int64_t lt_slot64(int64_t k) {
-    return ((k / kF64Depth) % 2) * kF64Depth + (k % kF64Depth);
+    const int D = f64_depth();
+    return ((k / D) % 2) * D + (k % D);
 }
 
 // C += L T over rows [r0, r0+M) and columns [c0, c0+N), steps k0 .. k0+K/64-1
@@ -101,7 +113,7 @@ hpl_status schedule64(hpl_ctx* c) {
     if (r) return r;
     for (int64_t k = 0; k + 1 < N; ++k) {
         const int64_t k1 = (k + 1) * B, m = n - k1;
-        const int64_t j = k % kF64Depth;
+        const int64_t j = k % f64_depth();
         const int K = (int)((j + 1) * B);
         const int64_t k0 = k - j;
         if (m <= B) {
@@ -114,7 +126,7 @@ hpl_status schedule64(hpl_ctx* c) {
         r = update64(c, k0, k1, k1, B, m, K);  // row block of the next panel
         if (!r) r = update64(c, k0, k1 + B, k1, m - B, B, K);  // column block
         if (!r) r = panel64(c, k + 1);
-        if (!r && j == kF64Depth - 1) r = update64(c, k0, k1 + B, k1 + B, m - B, m - B, K);
+        if (!r && j == f64_depth() - 1) r = update64(c, k0, k1 + B, k1 + B, m - B, m - B, K);
         if (r) return r;
     }
     return HPL_SUCCESS;
