@@ -16,6 +16,14 @@
 #include <cstdint>
 #include <limits>
 #include <random>
+#include <type_traits>
+
+// The element type, as the reference's HOST_DATA_TYPE: float by default,
+// double in the DATA_TYPE=double build (Linpack_hip_dp, -DHOST_DATA_TYPE=double),
+// whose factorisation and solve run on the FP64 calls of the C ABI.
+#ifndef HOST_DATA_TYPE
+#define HOST_DATA_TYPE float
+#endif
 
 #include "../../include/hpl_hip.h"
 #include "hpcc_base.hpp"
@@ -23,6 +31,8 @@
 namespace linpack {
 
 using hpcc_base::CommunicationType;
+using real_t = HOST_DATA_TYPE;
+constexpr bool kDoublePrecision = std::is_same<real_t, double>::value;
 
 // Error convention: a non-zero hpl_status becomes an exception, as ASSERT_CL
 // does in the reference (shared/setup/fpga_setup.cpp:112-122).
@@ -94,7 +104,7 @@ class LinpackProgramSettings : public hpcc_base::BaseSettings {
         map["Block Size"] = std::to_string(blockSize);
         map["Emulate"] = isEmulationKernel ? "Yes" : "No";
         map["Diagonally Dominant"] = isDiagonallyDominant ? "Yes" : "No";
-        map["Data Type"] = "cl_float";
+        map["Data Type"] = kDoublePrecision ? "cl_double" : "cl_float";
         map["FPGA Torus"] = "P=" + std::to_string(torus_width) + ", Q=" + std::to_string(torus_height);
         map["Generator"] = generator;
         return map;
@@ -103,14 +113,14 @@ class LinpackProgramSettings : public hpcc_base::BaseSettings {
 
 class LinpackData {
    public:
-    float* A = nullptr;
-    float* b = nullptr;
+    real_t* A = nullptr;
+    real_t* b = nullptr;
     int* ipvt = nullptr;
     size_t matrix_width, matrix_height;
-    float norma = 0.0f, normb = 0.0f;
+    real_t norma = 0, normb = 0;
     LinpackData(size_t width, size_t height) : matrix_width(width), matrix_height(height) {
-        if (posix_memalign(reinterpret_cast<void**>(&A), 4096, width * height * sizeof(float)) ||
-            posix_memalign(reinterpret_cast<void**>(&b), 4096, width * sizeof(float)) ||
+        if (posix_memalign(reinterpret_cast<void**>(&A), 4096, width * height * sizeof(real_t)) ||
+            posix_memalign(reinterpret_cast<void**>(&b), 4096, width * sizeof(real_t)) ||
             posix_memalign(reinterpret_cast<void**>(&ipvt), 4096, height * sizeof(int)))
             throw std::bad_alloc();
     }
@@ -141,9 +151,26 @@ inline std::map<std::string, std::vector<double>> calculate(const Settings& conf
     const uint32_t B = config.programSettings->blockSize;
     const bool pivoting = !config.programSettings->isDiagonallyDominant;
     hpl_timing t{};
+    if constexpr (kDoublePrecision) {
+        // DATA_TYPE=double: the FP64 system and calls (block 64)
+        ASSERT_HPL(c, hpl_upload_f64(c, reinterpret_cast<double*>(data.A), n, n,
+                                     reinterpret_cast<double*>(data.b), &t));
+        std::vector<double> gefa, gesl;
+        for (unsigned rep = 0; rep < config.programSettings->numRepetitions; ++rep) {
+            if (rep > 0) ASSERT_HPL(c, hpl_restore_f64(c));
+            ASSERT_HPL(c, hpl_gefa_dev_f64(c, &t));
+            ASSERT_HPL(c, hpl_gesl_dev_f64(c, &t));
+            gefa.push_back(t.gefa_s);
+            gesl.push_back(t.gesl_s);
+        }
+        ASSERT_HPL(c, hpl_download_f64(c, reinterpret_cast<double*>(data.A), n,
+                                       reinterpret_cast<double*>(data.b), &t));
+        return {{"gefa", gefa}, {"gesl", gesl}};
+    }
     ASSERT_HPL(c, hpl_keep_original(c, 1));
     ASSERT_HPL(c, hpl_set_pivoting(c, pivoting ? 1 : 0));
-    ASSERT_HPL(c, hpl_upload(c, data.A, n, n, data.b, &t));
+    ASSERT_HPL(c, hpl_upload(c, reinterpret_cast<float*>(data.A), n, n,
+                             reinterpret_cast<float*>(data.b), &t));
     std::vector<double> gefa, gesl;
     for (unsigned rep = 0; rep < config.programSettings->numRepetitions; ++rep) {
         if (rep > 0) ASSERT_HPL(c, hpl_restore(c));
@@ -152,7 +179,8 @@ inline std::map<std::string, std::vector<double>> calculate(const Settings& conf
         gefa.push_back(t.gefa_s);
         gesl.push_back(t.gesl_s);
     }
-    ASSERT_HPL(c, hpl_download(c, data.A, n, data.b, &t));
+    ASSERT_HPL(c, hpl_download(c, reinterpret_cast<float*>(data.A), n,
+                               reinterpret_cast<float*>(data.b), &t));
     if (pivoting) ASSERT_HPL(c, hpl_get_pivots(c, data.ipvt));
     return {{"gefa", gefa}, {"gesl", gesl}};
 }
@@ -196,6 +224,18 @@ class LinpackBenchmark
     bool checkInputParameters() override {
         auto& s = *executionSettings->programSettings;
         if (s.communicationType == CommunicationType::hip) {
+            if (kDoublePrecision && !s.isDiagonallyDominant) {
+                std::cerr << "ERROR: the DATA_TYPE=double build factorises without pivoting; "
+                             "--uniform needs the float build"
+                          << std::endl;
+                return false;
+            }
+            if (kDoublePrecision && s.matrixSize % 64) {
+                std::cerr << "ERROR: the DATA_TYPE=double build needs a matrix size that is a "
+                             "multiple of 64"
+                          << std::endl;
+                return false;
+            }
             if (!s.isDiagonallyDominant && s.blockSize > 256) {
                 std::cerr << "ERROR: partial pivoting (--uniform) needs a block size in "
                              "[32, 256] (b in 5..8)"
@@ -227,32 +267,32 @@ class LinpackBenchmark
         return d;
     }
 
-    // mt19937(rank) draws U(0,1) as double, stored as float row by row; a
-    // diagonally dominant matrix gets the float row sum on its diagonal; b is
-    // the float column sum, so the solution is all ones.
+    // mt19937(rank) draws U(0,1) as double, stored as HOST_DATA_TYPE row by
+    // row; a diagonally dominant matrix gets the row sum (in HOST_DATA_TYPE)
+    // on its diagonal; b is the column sum, so the solution is all ones.
     void generateReference(LinpackData& d, size_t n, bool dd) {
         std::mt19937 gen(mpi_comm_rank);
         std::uniform_real_distribution<> dis(0.0, 1.0);
-        float* A = d.A;
-        d.norma = 0.0f;
-        d.normb = 0.0f;
+        real_t* A = d.A;
+        d.norma = 0;
+        d.normb = 0;
         for (size_t j = 0; j < n; ++j)
             for (size_t i = 0; i < n; ++i) {
-                const float v = dis(gen);
+                const real_t v = dis(gen);
                 A[n * j + i] = v;
                 if (v > d.norma) d.norma = v;
             }
         if (dd) {
             for (size_t r = 0; r < n; ++r) {
-                A[n * r + r] = 0.0f;
-                float sum = 0.0f;
+                A[n * r + r] = 0;
+                real_t sum = 0;
                 for (size_t i = 0; i < n; ++i) sum += A[n * r + i];
                 if (sum > d.norma) d.norma = sum;
                 A[n * r + r] = sum;
             }
         }
         for (size_t j = 0; j < n; ++j) {
-            float sum = 0.0f;
+            real_t sum = 0;
             for (size_t i = 0; i < n; ++i) sum += A[n * i + j];
             d.b[j] = sum;
             if (sum > d.normb) d.normb = sum;
@@ -263,7 +303,8 @@ class LinpackBenchmark
     // hpl_generate mode 0, or mode 2 without dominance, produces the same
     // bits): A = U[-0.5,0.5) on a 2^-24 grid from splitmix64 of (seed, r*n+c),
     // A_ii += n when diagonally dominant, b = row sums of M in double rounded
-    // to float.
+    // to float (the DATA_TYPE=double build widens these float values, as
+    // hpl_generate_f64 does).
     static uint64_t mix64(uint64_t z) {
         z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
         z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
@@ -271,15 +312,15 @@ class LinpackBenchmark
     }
     void generateSized(LinpackData& d, size_t n, uint64_t seed, bool dd) {
         const uint64_t key = mix64(seed ^ (0ULL << 56));
-        float* A = d.A;
+        real_t* A = d.A;
 #pragma omp parallel for schedule(static)
         for (long long r = 0; r < (long long)n; ++r)
             for (size_t c = 0; c < n; ++c) {
                 const uint64_t z = mix64(key + ((uint64_t)r * n + c + 1) * 0x9E3779B97F4A7C15ULL);
-                A[r * n + c] = (float)(uint32_t)(z >> 40) * (1.0f / 16777216.0f) - 0.5f;
+                A[r * n + c] = (real_t)((float)(uint32_t)(z >> 40) * (1.0f / 16777216.0f) - 0.5f);
             }
         if (dd)
-            for (size_t i = 0; i < n; ++i) A[i * n + i] += (float)n;
+            for (size_t i = 0; i < n; ++i) A[i * n + i] = (real_t)((float)A[i * n + i] + (float)n);
         std::vector<double> s(n, 0.0);
         const int nt = omp_get_max_threads();
         const size_t chunk = (n + nt - 1) / nt;
@@ -292,12 +333,12 @@ class LinpackBenchmark
         d.norma = 0.0f;
         d.normb = 0.0f;
         for (size_t i = 0; i < n; ++i) {
-            d.b[i] = (float)s[i];
+            d.b[i] = (real_t)s[i];
             if (d.b[i] > d.normb) d.normb = d.b[i];
             if (dd && A[i * n + i] > d.norma) d.norma = A[i * n + i];
         }
         if (!dd) {  // the running maximum of the generated values, as the reference keeps it
-            float m = 0.0f;
+            real_t m = 0;
 #pragma omp parallel for reduction(max : m)
             for (long long e = 0; e < (long long)(n * n); ++e) m = std::max(m, A[e]);
             d.norma = m;
@@ -329,14 +370,15 @@ class LinpackBenchmark
             const double v = std::abs(data.b[i] - 1);
             resid = (resid > v) ? resid : v;
         }
-        const float eps = std::numeric_limits<float>::epsilon();
+        const real_t eps = std::numeric_limits<real_t>::epsilon();
         const double residn = resid / (static_cast<double>(n) * normx * eps);
         errors.emplace("epsilon", eps);
         errors.emplace("residual", resid);
         errors.emplace("residual_norm", residn);
         if (executionSettings->context) {
             double nrm[4];
-            if (hpl_residual(executionSettings->context->get(), nrm) == HPL_SUCCESS)
+            hpl_ctx* c = executionSettings->context->get();
+            if ((kDoublePrecision ? hpl_residual_f64(c, nrm) : hpl_residual(c, nrm)) == HPL_SUCCESS)
                 errors.emplace("hpl_scaled_residual", nrm[0] / (eps * nrm[1] * nrm[2] * n));
         }
         return residn < 1;

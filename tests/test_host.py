@@ -15,6 +15,7 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EXE = os.path.join(ROOT, "hpcc_fpga_amd", "host", "Linpack_hip")
+EXE_DP = os.path.join(ROOT, "hpcc_fpga_amd", "host", "Linpack_hip_dp")
 
 NUM = r"((\d|\.|\+|-|e)+|nan)"
 LINPACK_REGEX = (
@@ -120,3 +121,36 @@ def test_uniform_rejects_large_blocks():
     r = run("-f", "HIP", "-m", "2", "-b", "9", "--uniform", "--test")
     assert r.returncode == 1
     assert "partial pivoting (--uniform) needs a block size in [32, 256]" in r.stderr + r.stdout
+
+
+def test_double_build_parses():
+    """The DATA_TYPE=double build (Linpack_hip_dp) reports cl_double and
+    rejects what it cannot run."""
+    r = subprocess.run([EXE_DP, "-f", "HIP", "--test", "-m", "16", "-b", "6"], capture_output=True,
+                       text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    assert re.search(r"Data Type\s+cl_double", r.stdout)
+    r = subprocess.run([EXE_DP, "-f", "HIP", "--test", "-m", "3", "-b", "5"], capture_output=True,
+                       text=True, timeout=60)
+    assert r.returncode == 1
+    assert "multiple of 64" in r.stderr + r.stdout
+    r = subprocess.run([EXE_DP, "-f", "HIP", "--test", "-m", "16", "-b", "6", "--uniform"],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1
+    assert "without pivoting" in r.stderr + r.stdout
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("gen", ["ref", "sized"])
+def test_double_build_full_run(tmp_path, gen):
+    out = tmp_path / "dp.json"
+    print(f"host DATA_TYPE=double generator={gen}", flush=True)
+    r = subprocess.run([EXE_DP, "-f", "HIP", "-m", "16", "-b", "6", "-n", "2", "--generator", gen,
+                        "--dump-json", str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "Validation: SUCCESS!" in r.stdout
+    d = json.loads(out.read_text())
+    assert d["validated"] is True
+    assert d["errors"]["epsilon"] < 1e-15  # double epsilon
+    assert d["errors"]["residual_norm"] < 1
+    assert d["errors"]["hpl_scaled_residual"] < 1
