@@ -2,12 +2,13 @@
 // (its "_DP" build; gefa_ref_nopvt / gesl_ref_nopvt with HOST_DATA_TYPE
 // double, LINPACK/src/host/linpack_data.cpp:166-229).  Kernels in f64.hip.
 //
-// Schedule (stream s0): blocked right-looking LU with block 64 and a delayed
-// bulk update, as the FP32 schedule: at step k the next panel's row and
-// column blocks receive every pending update, the next panel is factorised
-// (diagonal LU, strip solves), and the rest of the trailing matrix is updated
-// every `depth` steps with K = depth * 64 -- K = 64 alone would leave the FP64
-// update at 8 flop per byte of C traffic, under the device's ~10.
+// Schedule: blocked right-looking LU with block 64, a depth-1 look-ahead and
+// a delayed bulk update, as the FP32 schedule: at step k the next panel's row
+// and column blocks receive every pending update, the next panel is
+// factorised (diagonal LU, strip solves) on the second stream, and the rest of
+// the trailing matrix is updated every `depth` steps with K = depth * 64 --
+// K = 64 alone would leave the FP64 update at 8 flop per byte of C traffic,
+// under the device's ~10.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -86,30 +87,35 @@ int64_t lt_slot64(int64_t k) {
 }
 
 // C += L T over rows [r0, r0+M) and columns [c0, c0+N), steps k0 .. k0+K/64-1
-hpl_status update64(hpl_ctx* c, int64_t k0, int64_t r0, int64_t c0, int64_t M, int64_t N, int K) {
+hpl_status update64(hpl_ctx* c, hipStream_t st, int64_t k0, int64_t r0, int64_t c0, int64_t M,
+                    int64_t N, int K) {
     if (M <= 0 || N <= 0) return HPL_SUCCESS;
     const int64_t n = (int64_t)c->n64, lda = (int64_t)c->lda64, B = f64_block();
     const double* Lt = c->dLt64 + lt_slot64(k0) * B * n + r0;
     const double* T = c->dA64 + k0 * B * lda + c0;
-    FCHK(c, launch_update_f64(c->dA64 + r0 * lda + c0, lda, Lt, n, T, lda, (int)M, (int)N, K,
-                              c->s0));
+    FCHK(c, launch_update_f64(c->dA64 + r0 * lda + c0, lda, Lt, n, T, lda, (int)M, (int)N, K, st));
     return HPL_SUCCESS;
 }
 
-hpl_status panel64(hpl_ctx* c, int64_t k) {
+hpl_status panel64(hpl_ctx* c, hipStream_t st, int64_t k) {
     const int64_t n = (int64_t)c->n64, lda = (int64_t)c->lda64, B = f64_block();
     const int64_t off = k * B;
-    FCHK(c, launch_lu_f64(c->dA64, lda, off, c->dStatus, c->s0));
+    FCHK(c, launch_lu_f64(c->dA64, lda, off, c->dStatus, st));
     const int64_t m = n - off - B;
     if (m > 0)
         FCHK(c, launch_strips_f64(c->dA64, lda, off, m, c->dLt64 + lt_slot64(k) * B * n + off + B,
-                                  n, c->s0));
+                                  n, st));
     return HPL_SUCCESS;
 }
 
+// Depth-1 look-ahead as the FP32 schedule: once the next panel's row and
+// column blocks are up to date (event evA), that panel is factorised on s1
+// while s0 runs the bulk update; s0 waits for it (evP) before the next step.
+// The panel writes its own Lt slot, which the bulk update of this step does
+// not read, and touches only its row and column blocks.
 hpl_status schedule64(hpl_ctx* c) {
     const int64_t n = (int64_t)c->n64, B = f64_block(), N = n / B;
-    hpl_status r = panel64(c, 0);
+    hpl_status r = panel64(c, c->s0, 0);
     if (r) return r;
     for (int64_t k = 0; k + 1 < N; ++k) {
         const int64_t k1 = (k + 1) * B, m = n - k1;
@@ -117,17 +123,24 @@ hpl_status schedule64(hpl_ctx* c) {
         const int K = (int)((j + 1) * B);
         const int64_t k0 = k - j;
         if (m <= B) {
-            r = update64(c, k0, k1, k1, m, m, K);
-            if (r) return r;
-            r = panel64(c, k + 1);
+            r = update64(c, c->s0, k0, k1, k1, m, m, K);
+            if (!r) r = panel64(c, c->s0, k + 1);
             if (r) return r;
             continue;
         }
-        r = update64(c, k0, k1, k1, B, m, K);  // row block of the next panel
-        if (!r) r = update64(c, k0, k1 + B, k1, m - B, B, K);  // column block
-        if (!r) r = panel64(c, k + 1);
-        if (!r && j == f64_depth() - 1) r = update64(c, k0, k1 + B, k1 + B, m - B, m - B, K);
+        r = update64(c, c->s0, k0, k1, k1, B, m, K);  // row block of the next panel
+        if (!r) r = update64(c, c->s0, k0, k1 + B, k1, m - B, B, K);  // column block
         if (r) return r;
+        FCHK(c, hipEventRecord(c->evA, c->s0));
+        FCHK(c, hipStreamWaitEvent(c->s1, c->evA, 0));
+        r = panel64(c, c->s1, k + 1);
+        if (r) return r;
+        FCHK(c, hipEventRecord(c->evP, c->s1));
+        if (j == f64_depth() - 1) {
+            r = update64(c, c->s0, k0, k1 + B, k1 + B, m - B, m - B, K);
+            if (r) return r;
+        }
+        FCHK(c, hipStreamWaitEvent(c->s0, c->evP, 0));
     }
     return HPL_SUCCESS;
 }
