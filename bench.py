@@ -37,10 +37,28 @@ def measured_traffic(n, B):
         return json.load(f)
 
 
-def cpu_baseline(n=5120, seed=42):
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
+def cpu_baseline(n=5120, seed=42, n_par=8192, full_n=32768):
     """The reference CPU path (gefa_ref_nopvt + gesl_ref_nopvt restated in the
-    oracle), one core, on a bounded sample of the same generator."""
+    oracle), one core, on a bounded sample of the same generator; beside it the
+    same loop nest with the independent columns of each step spread over the
+    host's cores (a parallel restatement, not the reference)."""
+    import numpy as np
+
     from oracle import oracle as O
+
+    def flops(m):
+        return 2.0 / 3.0 * m ** 3 + 2.0 * m * m
 
     A, b, _, _ = O.generate_sized(n, seed, 0)
     t0 = time.perf_counter()
@@ -48,19 +66,34 @@ def cpu_baseline(n=5120, seed=42):
     t1 = time.perf_counter()
     x = O.gesl_nopvt(F, b)
     t2 = time.perf_counter()
-    flops = 2.0 / 3.0 * n ** 3 + 2.0 * n * n
-    import numpy as np
-
     ok = float(np.max(np.abs(x - 1.0))) < 1e-3
+    one = flops(n) / (t2 - t0) / 1e9
+    # the host share of the box: at most 16 threads
+    threads = max(1, min(16, os.cpu_count() or 1))
+    Ap, bp, _, _ = O.generate_sized(n_par, seed, 0)
+    t3 = time.perf_counter()
+    Fp = O.gefa_nopvt_omp(Ap, threads)
+    xp = O.gesl_nopvt(Fp, bp)
+    t4 = time.perf_counter()
+    okp = float(np.max(np.abs(xp - 1.0))) < 1e-3
     return {
-        "value": flops / (t2 - t0) / 1e9,
+        "value": one,
         "unit": "GFLOP/s",
         "cores": 1,
         "kind": "port",
         "sample": f"gefa_ref_nopvt+gesl_ref_nopvt restated (oracle, g++ -O3 -march=x86-64-v3, "
                   f"1 thread), n={n} same generator seed {seed}; gefa {t1 - t0:.2f}s gesl "
-                  f"{t2 - t1:.3f}s; x=1 check {'ok' if ok else 'FAILED'}; cpu "
-                  f"{platform.processor() or platform.machine()}",
+                  f"{t2 - t1:.3f}s; x=1 check {'ok' if ok else 'FAILED'}; cpu {_cpu_model()}, "
+                  f"nproc {os.cpu_count()}",
+        "extrapolated_1core_s_at_n": {"n": full_n,
+                                      "seconds": round(flops(full_n) / (one * 1e9), 1),
+                                      "note": "extrapolated from the n above by n^3, not run"},
+        "parallel_restatement": {"gflops": round(flops(n_par) / (t4 - t3) / 1e9, 2),
+                                 "threads": threads, "n": n_par,
+                                 "seconds": round(t4 - t3, 2),
+                                 "x_check": "ok" if okp else "FAILED",
+                                 "note": "same loop nest, columns of each step over OpenMP "
+                                         "threads; not the reference (single threaded)"},
     }
 
 
