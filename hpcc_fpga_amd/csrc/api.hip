@@ -748,7 +748,11 @@ hpl_status hpl_gefa(hpl_ctx* c, float* a, uint64_t n, uint64_t lda, uint32_t blo
                     (unsigned long long)n, (unsigned long long)lda, block);
     hpl_status st = hpl_upload(c, a, n, lda, nullptr, t);
     if (st) return st;
+    // gefa_ref_nopvt's call: no pivoting whatever the context's setting
+    const bool saved = c->pivoting;
+    c->pivoting = false;
     st = hpl_gefa_dev(c, block, HPL_PREC_FP32, t);
+    c->pivoting = saved;
     if (st) return st;
     return hpl_download(c, a, lda, nullptr, t);
 }

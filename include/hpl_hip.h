@@ -134,7 +134,8 @@ hpl_status hpl_gesl_f64(hpl_ctx* ctx, const double* lu, double* b, uint64_t n, u
 hpl_status hpl_upload_f64(hpl_ctx* ctx, const double* a, uint64_t n, uint64_t lda,
                           const double* b, hpl_timing* timing);
 /* The system of hpl_generate(n, mode, seed, d) widened to FP64 (b = M x_true
- * in FP64 from the same values). */
+ * in FP64 from the same values); the FP32 resident system is that of
+ * hpl_generate afterwards. */
 hpl_status hpl_generate_f64(hpl_ctx* ctx, uint64_t n, int mode, uint64_t seed, double d);
 hpl_status hpl_gefa_dev_f64(hpl_ctx* ctx, hpl_timing* timing);
 hpl_status hpl_gesl_dev_f64(hpl_ctx* ctx, hpl_timing* timing);

@@ -189,6 +189,11 @@ def test_pivot_errors(ctx):
         assert e.value.status == 8
         with pytest.raises(HplError):
             c.pivots()  # nothing factorised yet
+        # the unpivoted host call ignores the context's pivoting setting
+        A3, _, _, _ = O.generate_ref(128, seed=2)
+        F3 = A3.copy()
+        c.gefa(F3, 64)
+        assert np.max(np.abs(F3 - O.gefa_nopvt(A3))) <= 2e-5 * np.max(np.abs(F3))
     finally:
         c.close()
     del ipvt
