@@ -25,12 +25,20 @@ struct hpl_ctx {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     hipEvent_t evA = nullptr, evP = nullptr;  // next-panel region updated / panel ready
     bool lookahead = true;
-    // Bulk trailing update every `depth` steps (K = depth * B).  FP32: 3 --
-    // at n = 32768, B = 256 the bulk launches run at 0.75 of the FP32 MFMA
-    // peak instead of 0.69-0.70 at depth 1 (fewer, longer launches that the
-    // look-ahead panel disturbs less) for about 0.6% more GEFA time; deeper
-    // is slower.  The 16-bit and FP8 updates are bound by the FP32 C traffic
-    // (8 bytes per element per pass), which `depth` divides: 8.
+    // Pipelined schedule (with look-ahead): the bulk update of one group of
+    // `depth` steps runs on s1 while the next group's panels run on s0; the
+    // transposed left strips then stay in one linear buffer (slot = step).
+    bool pipelined = true;
+    bool lt_linear = false;
+    // Column slices (separate launches) of each bulk update of the pipelined
+    // schedule; 0: as many as the update depth.
+    int bulk_slices = 0;
+    // Steps per bulk trailing update (K = depth * B).  FP32: 3 -- at
+    // n = 32768, B = 256 the bulk launches run at 0.75 of the FP32 MFMA peak
+    // instead of 0.69-0.70 at depth 1, and with the pipelined schedule depth
+    // 3 (207 ms) is within noise of 4 (207 ms) and ahead of 2 (210 ms).  The
+    // 16-bit and FP8 updates are bound by the FP32 C traffic (8 bytes per
+    // element per pass), which `depth` divides: 8.
     int depth = 1;
     int depth_fp32 = 3, depth_lp = 8;
     std::string err;
@@ -51,7 +59,7 @@ struct hpl_ctx {
 
     // GEFA workspaces
     uint32_t B = 0;             // block size of the resident factor (0: none)
-    float* dLt = nullptr;       // transposed left strips, 2 * depth slots of B x n
+    float* dLt = nullptr;       // transposed left strips, 2 * depth (linear: n / B) slots of B x n
     size_t capLt = 0;
     float* dDT = nullptr;       // transposed diagonal block, 2 x (B x B), by step parity
     size_t capDT = 0;
@@ -143,7 +151,9 @@ struct WsPlan {
     size_t inv;   // per-block 32 x 32 inverses (kept with the factor)
     size_t binv;  // full diagonal-block inverses for the substitution (kept with the factor)
 };
-WsPlan plan_factor_ws(uint64_t n, uint32_t B, int prec, int depth);
+// `linear`: one strip slot per step (the pipelined schedule) instead of
+// 2 * depth reused slots.
+WsPlan plan_factor_ws(uint64_t n, uint32_t B, int prec, int depth, bool linear = false);
 // Device bytes of the refinement buffers for nrhs columns of order n.
 struct IrPlan {
     size_t work, r, xs, out;

@@ -40,9 +40,26 @@ __global__ __launch_bounds__(NT) void lu_diag_kernel(float* __restrict__ A, int6
     const int lane = t & 63, wave = t >> 6;
     float* base = A + off * lda + off;
 
-    for (int e = t; e < B * B; e += NT) {
-        const int r = e / B, c = e % B;
-        Ds[r * LD + c] = base[(int64_t)r * lda + c];
+    // the block as float4s, every load issued before the first is used (one
+    // HBM round trip instead of one per element group); off is a multiple of
+    // 32 and lda of 4, so the rows are 16-byte aligned
+    constexpr int V = B * B / 4 / NT;
+    static_assert(V >= 1 && V * NT * 4 == B * B, "block must split evenly over the threads");
+    {
+        float4 v[V];
+#pragma unroll
+        for (int i = 0; i < V; ++i) {
+            const int e = t + i * NT, r = e / (B / 4), c = (e % (B / 4)) * 4;
+            v[i] = *reinterpret_cast<const float4*>(base + (int64_t)r * lda + c);
+        }
+#pragma unroll
+        for (int i = 0; i < V; ++i) {
+            const int e = t + i * NT, r = e / (B / 4), c = (e % (B / 4)) * 4;
+            Ds[r * LD + c] = v[i].x;
+            Ds[r * LD + c + 1] = v[i].y;
+            Ds[r * LD + c + 2] = v[i].z;
+            Ds[r * LD + c + 3] = v[i].w;
+        }
     }
     __syncthreads();
 
@@ -66,12 +83,22 @@ __global__ __launch_bounds__(NT) void lu_diag_kernel(float* __restrict__ A, int6
                 // scale pivot row j right of the diagonal; diagonal := -1/p
                 d0[j] = (lane > j) ? d0[j] * inv : ((lane == j) ? inv : d0[j]);
                 d1[j] = d1[j] * inv;
+                // the multipliers of column j (lane j's rows) reach every lane
+                // through ds_bpermute: vector results, so the 31 - j
+                // broadcasts pipeline without scalar registers or the
+                // readlane hazards
 #pragma unroll
                 for (int r = j + 1; r < 32; ++r) {
-                    const float s = bcast(d0[r], j);
-                    if (lane > j) d0[r] = d0[r] + d0[j] * s;
+                    const float s = __shfl(d0[r], j);
+                    const float u = d0[r] + d0[j] * s;
+                    d0[r] = (lane > j) ? u : d0[r];  // a select, not a branch
                     d1[r] = d1[r] + d1[j] * s;
                 }
+                // finish this step's updates here: the compiler would
+                // otherwise delay those of d1 (off the pivot chain) to the
+                // end and keep every broadcast value alive until then
+#pragma unroll
+                for (int r = 0; r < 32; ++r) asm volatile("" : "+v"(d0[r]), "+v"(d1[r]));
             }
 #pragma unroll
             for (int r = 0; r < 32; ++r) {
@@ -126,9 +153,11 @@ __global__ __launch_bounds__(NT) void lu_diag_kernel(float* __restrict__ A, int6
         __syncthreads();
     }
 
-    for (int e = t; e < B * B; e += NT) {
-        const int r = e / B, c = e % B;
-        base[(int64_t)r * lda + c] = Ds[r * LD + c];
+#pragma unroll
+    for (int i = 0; i < V; ++i) {
+        const int e = t + i * NT, r = e / (B / 4), c = (e % (B / 4)) * 4;
+        *reinterpret_cast<float4*>(base + (int64_t)r * lda + c) =
+            float4{Ds[r * LD + c], Ds[r * LD + c + 1], Ds[r * LD + c + 2], Ds[r * LD + c + 3]};
     }
     if (bad && status) atomicMax(status, 1);
 }
@@ -137,6 +166,9 @@ __global__ __launch_bounds__(NT) void lu_diag_kernel(float* __restrict__ A, int6
 
 hipError_t launch_lu_diag(float* A, int64_t lda, int64_t off, int B, int* status,
                           hipStream_t s) {
+    // 16-byte row vectors
+    if ((lda & 3) || (off & 3) || (reinterpret_cast<uintptr_t>(A) & 15))
+        return hipErrorInvalidValue;
     switch (B) {
         case 32: lu_diag_kernel<32><<<1, NT, 0, s>>>(A, lda, off, status); break;
         case 64: lu_diag_kernel<64><<<1, NT, 0, s>>>(A, lda, off, status); break;

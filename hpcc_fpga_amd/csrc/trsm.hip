@@ -50,14 +50,30 @@ __global__ __launch_bounds__(64) void prep_kernel(const float* __restrict__ A, i
     const float* D = A + off * lda + off;
     inv += (int64_t)blockIdx.y * inv_stride;
     if (gridDim.y > 1) D += (int64_t)blockIdx.y * (B * lda + B);  // all-blocks mode
+    // a 32 x 32 sub-block into S: four float4 per thread, all issued before
+    // the first is used (rows are 16-byte aligned: off is a multiple of 32,
+    // lda of 4)
+    auto load_sub = [&](int r0, int c0) {
+        float4 v[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int e = t + 64 * k, i = e / 8, j = (e % 8) * 4;
+            v[k] = *reinterpret_cast<const float4*>(D + (int64_t)(r0 + i) * lda + c0 + j);
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int e = t + 64 * k, i = e / 8, j = (e % 8) * 4;
+            S[i][j] = v[k].x;
+            S[i][j + 1] = v[k].y;
+            S[i][j + 2] = v[k].z;
+            S[i][j + 3] = v[k].w;
+        }
+    };
     if (blk >= 3 * NQ) {
         if (DT == nullptr) return;
         blk -= 3 * NQ;
         const int tr = blk / NQ, tc = blk % NQ;
-        for (int e = t; e < 1024; e += 64) {
-            const int i = e / 32, j = e % 32;
-            S[i][j] = D[(int64_t)(tr * 32 + i) * lda + tc * 32 + j];
-        }
+        load_sub(tr * 32, tc * 32);
         __syncthreads();
         for (int e = t; e < 1024; e += 64) {
             const int j = e / 32, i = e % 32;
@@ -66,10 +82,7 @@ __global__ __launch_bounds__(64) void prep_kernel(const float* __restrict__ A, i
         return;
     }
     const int type = blk / NQ, q = blk % NQ;
-    for (int e = t; e < 1024; e += 64) {
-        const int i = e / 32, j = e % 32;
-        S[i][j] = D[(int64_t)(q * 32 + i) * lda + q * 32 + j];
-    }
+    load_sub(q * 32, q * 32);
     __syncthreads();
     if (t < 32) {
         const int j = t;
@@ -508,6 +521,8 @@ hipError_t launch_block_inverse(const float* A, int64_t lda, int64_t n, int B, c
 
 hipError_t launch_prep(const float* A, int64_t lda, int64_t off, int B, float* DT, float* inv,
                        hipStream_t s) {
+    if ((lda & 3) || (off & 3) || (reinterpret_cast<uintptr_t>(A) & 15))
+        return hipErrorInvalidValue;  // 16-byte row vectors
     const int NQ = B / 32;
     dim3 grid(3 * NQ + (DT ? NQ * NQ : 0), 1);
     prep_kernel<<<grid, 64, 0, s>>>(A, lda, off, B, DT, inv, 0);
@@ -516,6 +531,7 @@ hipError_t launch_prep(const float* A, int64_t lda, int64_t off, int B, float* D
 
 hipError_t launch_prep_all(const float* A, int64_t lda, int B, int nblocks, float* inv,
                            int64_t inv_stride, hipStream_t s) {
+    if ((lda & 3) || (reinterpret_cast<uintptr_t>(A) & 15)) return hipErrorInvalidValue;
     const int NQ = B / 32;
     dim3 grid(3 * NQ, nblocks);
     prep_kernel<<<grid, 64, 0, s>>>(A, lda, 0, B, nullptr, inv, inv_stride);
