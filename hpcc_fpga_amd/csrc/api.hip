@@ -614,8 +614,9 @@ hpl_status hpl_create(int device, hpl_ctx** out) {
     if (const char* e = getenv("HPL_LOOKAHEAD")) c->lookahead = atoi(e) != 0;
     if (const char* e = getenv("HPL_SCHEDULE")) c->pipelined = strcmp(e, "grouped") != 0;
     if (const char* e = getenv("HPL_BULK_SLICES")) c->bulk_slices = std::max(0, atoi(e));
+    if (const char* e = getenv("HPL_F64_DEPTH")) c->depth_f64 = std::max(1, std::min(16, atoi(e)));
     if (const char* e = getenv("HPL_UPDATE_DEPTH"))
-        c->depth_fp32 = c->depth_lp = std::max(1, std::min(8, atoi(e)));
+        c->depth_fp32 = c->depth_lp = std::max(1, std::min(16, atoi(e)));
     if (hplx::init_streams(c) != HPL_SUCCESS) {
         hpl_destroy(c);
         return HPL_ERR_HIP;

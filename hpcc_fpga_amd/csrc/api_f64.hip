@@ -2,13 +2,13 @@
 // (its "_DP" build; gefa_ref_nopvt / gesl_ref_nopvt with HOST_DATA_TYPE
 // double, LINPACK/src/host/linpack_data.cpp:166-229).  Kernels in f64.hip.
 //
-// Schedule: blocked right-looking LU with block 64, a depth-1 look-ahead and
-// a delayed bulk update, as the FP32 schedule: at step k the next panel's row
-// and column blocks receive every pending update, the next panel is
-// factorised (diagonal LU, strip solves) on the second stream, and the rest of
-// the trailing matrix is updated every `depth` steps with K = depth * 64 --
-// K = 64 alone would leave the FP64 update at 8 flop per byte of C traffic,
-// under the device's ~10.
+// Schedule: blocked right-looking LU with block 64 and the FP32 path's two
+// schedules: pipelined (default) -- the panels of one group of `depth` steps
+// and their row/column-block updates on s0 while s1 applies the previous
+// group's steps (K = depth * 64) to the trailing matrix beyond, in column
+// slices -- or grouped (HPL_SCHEDULE=grouped: depth-1 look-ahead with a bulk
+// update every `depth` steps).  K = 64 alone would leave the FP64 update at 8
+// flop per byte of C traffic, under the device's ~10.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -22,18 +22,9 @@ namespace {
 
 using hplx::fail;
 
-// Update depth: the bulk update runs every depth steps with K = depth * 64.
-// Measured at n = 32768: depth 1/2/3/4/6/8 -> 27.3/35.0/39.1/41.3/43.5/44.4
-// TFLOP/s (n = 16384 peaks at 4-6); 6 by default, HPL_F64_DEPTH (1..8)
-// overrides it.
-constexpr int kF64DepthMax = 8;
-int f64_depth() {
-    static const int d = [] {
-        const char* e = getenv("HPL_F64_DEPTH");
-        return e ? std::max(1, std::min(kF64DepthMax, atoi(e))) : 6;
-    }();
-    return d;
-}
+// Update depth (steps per bulk update, K = depth * 64): the context's
+// depth_f64.
+int f64_depth(const hpl_ctx* c) { return c->depth_f64; }
 
 #define FCHK(ctx, expr)                                                                   \
     do {                                                                                  \
@@ -68,9 +59,6 @@ hpl_status alloc_f64(hpl_ctx* c, uint64_t n) {
         if (!st) st = grow(c, c->dOut64, cap, 4 * sizeof(double));
         if (!st) c->capV64 = n * sizeof(double);
     }
-    if (!st)
-        st = grow(c, c->dLt64, c->capLt64,
-                  2 * (size_t)kF64DepthMax * f64_block() * n * sizeof(double));
     if (!st) st = grow(c, c->dWork64, c->capWork64, residual_f64_work((int64_t)n) * sizeof(double));
     if (!st) st = hplx::ensure_mem(c, reinterpret_cast<void**>(&c->dStatus), &c->capStatus, 64);
     if (st) return st;
@@ -81,8 +69,12 @@ hpl_status alloc_f64(hpl_ctx* c, uint64_t n) {
     return HPL_SUCCESS;
 }
 
-int64_t lt_slot64(int64_t k) {
-    const int D = f64_depth();
+// Slot of step k's transposed left strip (64 x n): one per step in the
+// pipelined schedule (any window of steps is contiguous), else two groups of
+// `depth` reused in turn.
+int64_t lt_slot64(const hpl_ctx* c, int64_t k) {
+    if (c->f64_linear) return k;
+    const int D = f64_depth(c);
     return ((k / D) % 2) * D + (k % D);
 }
 
@@ -91,7 +83,7 @@ hpl_status update64(hpl_ctx* c, hipStream_t st, int64_t k0, int64_t r0, int64_t 
                     int64_t N, int K) {
     if (M <= 0 || N <= 0) return HPL_SUCCESS;
     const int64_t n = (int64_t)c->n64, lda = (int64_t)c->lda64, B = f64_block();
-    const double* Lt = c->dLt64 + lt_slot64(k0) * B * n + r0;
+    const double* Lt = c->dLt64 + lt_slot64(c, k0) * B * n + r0;
     const double* T = c->dA64 + k0 * B * lda + c0;
     FCHK(c, launch_update_f64(c->dA64 + r0 * lda + c0, lda, Lt, n, T, lda, (int)M, (int)N, K, st));
     return HPL_SUCCESS;
@@ -103,8 +95,8 @@ hpl_status panel64(hpl_ctx* c, hipStream_t st, int64_t k) {
     FCHK(c, launch_lu_f64(c->dA64, lda, off, c->dStatus, st));
     const int64_t m = n - off - B;
     if (m > 0)
-        FCHK(c, launch_strips_f64(c->dA64, lda, off, m, c->dLt64 + lt_slot64(k) * B * n + off + B,
-                                  n, st));
+        FCHK(c, launch_strips_f64(c->dA64, lda, off, m,
+                                  c->dLt64 + lt_slot64(c, k) * B * n + off + B, n, st));
     return HPL_SUCCESS;
 }
 
@@ -119,7 +111,7 @@ hpl_status schedule64(hpl_ctx* c) {
     if (r) return r;
     for (int64_t k = 0; k + 1 < N; ++k) {
         const int64_t k1 = (k + 1) * B, m = n - k1;
-        const int64_t j = k % f64_depth();
+        const int64_t j = k % f64_depth(c);
         const int K = (int)((j + 1) * B);
         const int64_t k0 = k - j;
         if (m <= B) {
@@ -136,10 +128,52 @@ hpl_status schedule64(hpl_ctx* c) {
         r = panel64(c, c->s1, k + 1);
         if (r) return r;
         FCHK(c, hipEventRecord(c->evP, c->s1));
-        if (j == f64_depth() - 1) {
+        if (j == f64_depth(c) - 1) {
             r = update64(c, c->s0, k0, k1 + B, k1 + B, m - B, m - B, K);
             if (r) return r;
         }
+        FCHK(c, hipStreamWaitEvent(c->s0, c->evP, 0));
+    }
+    return HPL_SUCCESS;
+}
+
+// Pipelined schedule, as the FP32 one (gefa_pipelined in api.hip): phase g
+// factorises the blocks of group g on s0, each after the update of its row
+// and column blocks by the steps of groups g-1 and g before it, while s1
+// applies group g-1's steps to rows and columns >= (g+1) depth 64 in column
+// slices; the streams join at the end of the phase.
+hpl_status schedule64_pipelined(hpl_ctx* c) {
+    const int64_t n = (int64_t)c->n64, B = f64_block(), N = n / B;
+    const int64_t D = f64_depth(c), G = (N + D - 1) / D;
+    const int64_t slices = c->bulk_slices > 0 ? c->bulk_slices : D;
+    hpl_status r;
+    for (int64_t g = 0; g < G; ++g) {
+        const int64_t s = (g + 1) * D * B;
+        if (g >= 1 && s < n) {
+            const int64_t w = n - s, tiles = w / 128;
+            const int64_t S = std::max<int64_t>(1, std::min(slices, tiles));
+            for (int64_t i = 0; i < S; ++i) {
+                const int64_t a = s + tiles * i / S * 128;
+                const int64_t b = i + 1 == S ? n : s + tiles * (i + 1) / S * 128;
+                r = update64(c, c->s1, (g - 1) * D, s, a, w, b - a, (int)(D * B));
+                if (r) return r;
+            }
+        }
+        FCHK(c, hipEventRecord(c->evP, c->s1));
+        const int64_t k0 = std::max<int64_t>(0, (g - 1) * D);
+        for (int64_t k = g * D; k < std::min(N, (g + 1) * D); ++k) {
+            const int64_t kb = k * B, m = n - kb;
+            if (k > k0) {
+                const int K = (int)((k - k0) * B);
+                r = update64(c, c->s0, k0, kb, kb, B, m, K);  // row block k
+                if (!r) r = update64(c, c->s0, k0, kb + B, kb, m - B, B, K);  // column block k
+                if (r) return r;
+            }
+            r = panel64(c, c->s0, k);
+            if (r) return r;
+        }
+        FCHK(c, hipEventRecord(c->evA, c->s0));
+        FCHK(c, hipStreamWaitEvent(c->s1, c->evA, 0));
         FCHK(c, hipStreamWaitEvent(c->s0, c->evP, 0));
     }
     return HPL_SUCCESS;
@@ -212,8 +246,14 @@ hpl_status hpl_gefa_dev_f64(hpl_ctx* c, hpl_timing* t) {
     FCHK(c, hipSetDevice(c->device));
     FCHK(c, hipMemsetAsync(c->dStatus, 0, sizeof(int), c->s0));
     c->factored64 = false;
+    // strip slots: one per step (pipelined) or 2 * depth
+    c->f64_linear = c->pipelined && c->lookahead;
+    const uint64_t n = c->n64;
+    const size_t slots = c->f64_linear ? n / f64_block() : 2 * (size_t)f64_depth(c);
+    hpl_status st = grow(c, c->dLt64, c->capLt64, slots * f64_block() * n * sizeof(double));
+    if (st) return st;
     FCHK(c, hipEventRecord(c->e0, c->s0));
-    hpl_status st = schedule64(c);
+    st = c->f64_linear ? schedule64_pipelined(c) : schedule64(c);
     if (st) return st;
     FCHK(c, hipEventRecord(c->e1, c->s0));
     FCHK(c, hipStreamSynchronize(c->s0));

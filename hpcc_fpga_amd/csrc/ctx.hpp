@@ -38,9 +38,10 @@ struct hpl_ctx {
     // instead of 0.69-0.70 at depth 1, and with the pipelined schedule depth
     // 3 (207 ms) is within noise of 4 (207 ms) and ahead of 2 (210 ms).  The
     // 16-bit and FP8 updates are bound by the FP32 C traffic (8 bytes per
-    // element per pass), which `depth` divides: 8.
+    // element per pass), which `depth` divides: 12 (n = 65536, BF16: 327 ms
+    // against 334 at 8 and 329 at 16; FP8: 299 against 314 at 8).
     int depth = 1;
-    int depth_fp32 = 3, depth_lp = 8;
+    int depth_fp32 = 3, depth_lp = 12;
     std::string err;
 
     // resident system
@@ -117,7 +118,7 @@ struct hpl_ctx {
     uint32_t pv_B = 0;
 
     // FP64 system (api_f64.hip): matrix, kept original, b, x, transposed
-    // left strips (2 * kF64Depth slots of 64 x n) and the residual partials
+    // left strips (slots of 64 x n) and the residual partials
     double* dA64 = nullptr;
     double* dA064 = nullptr;
     double* dB64s = nullptr;
@@ -127,6 +128,10 @@ struct hpl_ctx {
     double* dOut64 = nullptr;
     size_t capA64 = 0, capA064 = 0, capV64 = 0, capLt64 = 0, capWork64 = 0;
     uint64_t n64 = 0, lda64 = 0;
+    // Steps per FP64 bulk update (K = depth_f64 * 64); f64_linear: one strip
+    // slot per step (pipelined schedule) instead of 2 * depth_f64 reused.
+    int depth_f64 = 6;
+    bool f64_linear = false;
     bool have_orig64 = false;
     bool factored64 = false;
 
