@@ -1173,4 +1173,32 @@ hpl_status hpl_kernel_stats(hpl_ctx* c, uint64_t* launches, double* seconds, dou
     return HPL_SUCCESS;
 }
 
+hpl_status hpl_kernel_busy(hpl_ctx* c, double* seconds) {
+    if (!c || !seconds) return HPL_ERR_INVALID_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->s0));
+    HIPCHK(c, hipStreamSynchronize(c->s1));
+    std::vector<std::pair<double, double>> iv;
+    iv.reserve(c->ev_used);
+    for (size_t i = 0; i < c->ev_used; ++i) {
+        float a = 0.f, b = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&a, c->ev[0].first, c->ev[i].first));
+        HIPCHK(c, hipEventElapsedTime(&b, c->ev[0].first, c->ev[i].second));
+        iv.emplace_back(a * 1e-3, b * 1e-3);
+    }
+    std::sort(iv.begin(), iv.end());
+    double busy = 0.0, lo = 0.0, hi = -1.0;
+    for (const auto& x : iv) {
+        if (x.first > hi) {
+            if (hi > lo) busy += hi - lo;
+            lo = x.first;
+            hi = x.second;
+        } else {
+            hi = std::max(hi, x.second);
+        }
+    }
+    if (hi > lo) busy += hi - lo;
+    *seconds = busy;
+    return HPL_SUCCESS;
+}
+
 }  // extern "C"

@@ -160,6 +160,7 @@ def load_library():
         "hpl_residual_f64": ([P, C.POINTER(C.c_double)], i32),
         "hpl_set_profiling": ([P, i32], i32),
         "hpl_kernel_stats": ([P, C.POINTER(u64), C.POINTER(dbl), C.POINTER(dbl)], i32),
+        "hpl_kernel_busy": ([P, C.POINTER(dbl)], i32),
     }
     for name, (args, res) in sigs.items():
         fn = getattr(L, name)
@@ -440,6 +441,12 @@ class Context:
         n, s, f = C.c_uint64(), C.c_double(), C.c_double()
         self._check(self._L.hpl_kernel_stats(self._c, C.byref(n), C.byref(s), C.byref(f)))
         return int(n.value), float(s.value), float(f.value)
+
+    def kernel_busy(self) -> float:
+        """Seconds during which at least one profiled trailing-update launch ran."""
+        s = C.c_double()
+        self._check(self._L.hpl_kernel_busy(self._c, C.byref(s)))
+        return float(s.value)
 
 
 class Batch:

@@ -350,6 +350,10 @@ hpl_status hpl_serve_stats_get(hpl_server* server, hpl_serve_stats* stats);
  * flops (2*m_rows*m_cols*B per launch). */
 hpl_status hpl_set_profiling(hpl_ctx* ctx, int enable);
 hpl_status hpl_kernel_stats(hpl_ctx* ctx, uint64_t* launches, double* seconds, double* flops);
+/* Time (s) during which at least one of those launches ran: the union of
+ * their intervals.  Launches on the two streams overlap, so this is below
+ * the summed duration; flops / busy is the device's trailing-update rate. */
+hpl_status hpl_kernel_busy(hpl_ctx* ctx, double* seconds);
 
 #ifdef __cplusplus
 }
