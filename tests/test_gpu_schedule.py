@@ -1,0 +1,91 @@
+"""GPU: the pipelined GEFA schedule gives the grouped schedule's factor.
+
+Both schedules apply every step's update to each element exactly once, in
+step order, through the same GEMM kernels; they only split the steps into
+launches differently (windows of whole steps, so the K splits fall on block
+boundaries and every MFMA sees the same operand pairs).  Each element is
+therefore formed by the same floating-point operations in the same order,
+and the factors must agree bit for bit.  The grouped schedule is the one the
+earlier parity tests pinned against the oracle; the cases below cover group
+counts that do not divide the number of blocks, more slices than steps per
+group, the 16-bit update and the FP64 path.
+
+The schedule, update depth and slice count are read when a context is
+created (HPL_SCHEDULE, HPL_UPDATE_DEPTH, HPL_BULK_SLICES, HPL_F64_DEPTH).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from hpcc_fpga_amd import Context
+
+pytestmark = pytest.mark.gpu
+
+KEYS = ("HPL_SCHEDULE", "HPL_UPDATE_DEPTH", "HPL_BULK_SLICES", "HPL_F64_DEPTH")
+
+
+def _context(env):
+    saved = {k: os.environ.get(k) for k in KEYS}
+    for k in KEYS:
+        os.environ.pop(k, None)
+    os.environ.update(env)
+    try:
+        return Context(0)
+    finally:
+        for k, v in saved.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+
+
+def _factor(env, n, B, prec="fp32", seed=5):
+    print(f"schedule {env} n={n} B={B} {prec}", flush=True)
+    c = _context(env)
+    try:
+        c.generate(n, 0, seed)
+        c.gefa_dev(B, prec)
+        a, _ = c.download(matrix=True, solution=False)
+        return a
+    finally:
+        c.close()
+
+
+@pytest.mark.parametrize("n,B,depth,slices", [
+    (2048, 128, 3, 0),   # 16 blocks: groups of 3, the last one short
+    (1792, 256, 3, 0),   # 7 blocks
+    (2560, 128, 4, 3),   # fewer slices than steps per group
+    (1024, 64, 2, 5),    # more slices than steps per group
+    (4096, 256, 5, 0),
+])
+def test_pipelined_equals_grouped_fp32(n, B, depth, slices):
+    g = _factor({"HPL_SCHEDULE": "grouped", "HPL_UPDATE_DEPTH": str(depth)}, n, B)
+    p = _factor({"HPL_SCHEDULE": "pipelined", "HPL_UPDATE_DEPTH": str(depth),
+                 "HPL_BULK_SLICES": str(slices)}, n, B)
+    assert np.array_equal(g.view(np.uint32), p.view(np.uint32))
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp8"])
+def test_pipelined_equals_grouped_low_precision(prec):
+    n, B = 3072, 256
+    g = _factor({"HPL_SCHEDULE": "grouped", "HPL_UPDATE_DEPTH": "5"}, n, B, prec)
+    p = _factor({"HPL_SCHEDULE": "pipelined", "HPL_UPDATE_DEPTH": "5"}, n, B, prec)
+    assert np.array_equal(g.view(np.uint32), p.view(np.uint32))
+
+
+def test_pipelined_equals_grouped_fp64():
+    n = 1600  # 25 blocks of 64: groups of 6, the last one short
+    out = []
+    for env in ({"HPL_SCHEDULE": "grouped", "HPL_F64_DEPTH": "6"},
+                {"HPL_SCHEDULE": "pipelined", "HPL_F64_DEPTH": "6"}):
+        print(f"schedule {env} FP64 n={n}", flush=True)
+        c = _context(env)
+        try:
+            c.generate_f64(n, 0, 3)
+            c.gefa_dev_f64()
+            a, _ = c.download_f64(matrix=True, solution=False)
+            out.append(a)
+        finally:
+            c.close()
+    assert np.array_equal(out[0].view(np.uint64), out[1].view(np.uint64))
