@@ -369,30 +369,34 @@ hpl_status gefa_schedule(hpl_ctx* c, uint32_t B) {
 // B = 256, D = 3 measured 207 ms against 225 ms for the grouped schedule and
 // 221 ms for this one with single bulk launches (D = 4: 207 ms, D = 2:
 // 210 ms; 2 slices at D = 3: 215 ms).
+//
+// Groups need not be equal: hplx::group_bounds gives their boundaries, so the
+// last part of the factorisation can use shorter groups (tail_m, tail_depth).
 hpl_status gefa_pipelined(hpl_ctx* c, uint32_t B) {
     const int64_t n = (int64_t)c->n;
     const int64_t N = n / B;
-    const int64_t D = c->depth;
-    const int64_t G = (N + D - 1) / D;
-    const int64_t slices = c->bulk_slices > 0 ? c->bulk_slices : D;
+    const std::vector<int64_t> bnd = hplx::group_bounds(c, N, B, c->depth);
+    const int64_t G = (int64_t)bnd.size() - 1;
     hpl_status r;
     for (int64_t g = 0; g < G; ++g) {
-        const int64_t s = (g + 1) * D * (int64_t)B;
+        const int64_t s = bnd[g + 1] * (int64_t)B;
         if (g >= 1 && s < n) {
-            // column slices of whole 128-wide tiles
+            // group g-1's steps in column slices of whole 128-wide tiles
+            const int64_t steps = bnd[g] - bnd[g - 1];
+            const int64_t slices = c->bulk_slices > 0 ? c->bulk_slices : steps;
             const int64_t w = n - s, tiles = w / 128;
             const int64_t S = std::max<int64_t>(1, std::min(slices, tiles));
             for (int64_t i = 0; i < S; ++i) {
                 const int64_t a = s + tiles * i / S * 128;
                 const int64_t b = i + 1 == S ? n : s + tiles * (i + 1) / S * 128;
-                r = launch_update(c, c->s1, (g - 1) * D, s, a, (int)w, (int)(b - a),
-                                  (int)(D * B), B);
+                r = launch_update(c, c->s1, bnd[g - 1], s, a, (int)w, (int)(b - a),
+                                  (int)(steps * B), B);
                 if (r) return r;
             }
         }
         HIPCHK(c, hipEventRecord(c->evP, c->s1));
-        const int64_t k0 = std::max<int64_t>(0, (g - 1) * D);
-        for (int64_t k = g * D; k < std::min(N, (g + 1) * D); ++k) {
+        const int64_t k0 = g >= 1 ? bnd[g - 1] : 0;
+        for (int64_t k = bnd[g]; k < bnd[g + 1]; ++k) {
             const int64_t kb = k * (int64_t)B, m = n - kb;
             if (k > k0) {
                 r = launch_update2(c, c->s0, k0, UpdRegion{kb, kb, (int)B, (int)m},
@@ -431,6 +435,19 @@ double elapsed_s(hpl_ctx* c) {
 }  // namespace
 
 namespace hplx {
+
+std::vector<int64_t> group_bounds(const hpl_ctx* c, int64_t N, int64_t B, int64_t depth) {
+    const int64_t n = N * B;
+    const int64_t tail_m = c->tail_m < 0 ? n / 2 : c->tail_m;
+    const int64_t tail_d = c->tail_depth > 0 ? c->tail_depth : std::max<int64_t>(1, depth / 2);
+    std::vector<int64_t> b{0};
+    while (b.back() < N) {
+        const bool tail = tail_m > 0 && n - b.back() * B <= tail_m;
+        const int64_t d = std::max<int64_t>(1, tail ? tail_d : depth);
+        b.push_back(std::min(N, b.back() + d));
+    }
+    return b;
+}
 
 uint64_t device_lda(uint64_t n) {
     uint64_t lda = (n + 63) / 64 * 64;
@@ -614,6 +631,8 @@ hpl_status hpl_create(int device, hpl_ctx** out) {
     if (const char* e = getenv("HPL_LOOKAHEAD")) c->lookahead = atoi(e) != 0;
     if (const char* e = getenv("HPL_SCHEDULE")) c->pipelined = strcmp(e, "grouped") != 0;
     if (const char* e = getenv("HPL_BULK_SLICES")) c->bulk_slices = std::max(0, atoi(e));
+    if (const char* e = getenv("HPL_TAIL_M")) c->tail_m = std::max(0LL, atoll(e));
+    if (const char* e = getenv("HPL_TAIL_DEPTH")) c->tail_depth = std::max(0, std::min(16, atoi(e)));
     if (const char* e = getenv("HPL_F64_DEPTH")) c->depth_f64 = std::max(1, std::min(16, atoi(e)));
     if (const char* e = getenv("HPL_UPDATE_DEPTH"))
         c->depth_fp32 = c->depth_lp = std::max(1, std::min(16, atoi(e)));

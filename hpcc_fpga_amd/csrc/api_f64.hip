@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <cstdlib>
+#include <vector>
 
 #include "../../include/hpl_hip.h"
 #include "ctx.hpp"
@@ -144,24 +145,26 @@ hpl_status schedule64(hpl_ctx* c) {
 // slices; the streams join at the end of the phase.
 hpl_status schedule64_pipelined(hpl_ctx* c) {
     const int64_t n = (int64_t)c->n64, B = f64_block(), N = n / B;
-    const int64_t D = f64_depth(c), G = (N + D - 1) / D;
-    const int64_t slices = c->bulk_slices > 0 ? c->bulk_slices : D;
+    const std::vector<int64_t> bnd = hplx::group_bounds(c, N, B, f64_depth(c));
+    const int64_t G = (int64_t)bnd.size() - 1;
     hpl_status r;
     for (int64_t g = 0; g < G; ++g) {
-        const int64_t s = (g + 1) * D * B;
+        const int64_t s = bnd[g + 1] * B;
         if (g >= 1 && s < n) {
+            const int64_t steps = bnd[g] - bnd[g - 1];
+            const int64_t slices = c->bulk_slices > 0 ? c->bulk_slices : steps;
             const int64_t w = n - s, tiles = w / 128;
             const int64_t S = std::max<int64_t>(1, std::min(slices, tiles));
             for (int64_t i = 0; i < S; ++i) {
                 const int64_t a = s + tiles * i / S * 128;
                 const int64_t b = i + 1 == S ? n : s + tiles * (i + 1) / S * 128;
-                r = update64(c, c->s1, (g - 1) * D, s, a, w, b - a, (int)(D * B));
+                r = update64(c, c->s1, bnd[g - 1], s, a, w, b - a, (int)(steps * B));
                 if (r) return r;
             }
         }
         FCHK(c, hipEventRecord(c->evP, c->s1));
-        const int64_t k0 = std::max<int64_t>(0, (g - 1) * D);
-        for (int64_t k = g * D; k < std::min(N, (g + 1) * D); ++k) {
+        const int64_t k0 = g >= 1 ? bnd[g - 1] : 0;
+        for (int64_t k = bnd[g]; k < bnd[g + 1]; ++k) {
             const int64_t kb = k * B, m = n - kb;
             if (k > k0) {
                 const int K = (int)((k - k0) * B);

@@ -10,8 +10,9 @@ earlier parity tests pinned against the oracle; the cases below cover group
 counts that do not divide the number of blocks, more slices than steps per
 group, the 16-bit update and the FP64 path.
 
-The schedule, update depth and slice count are read when a context is
-created (HPL_SCHEDULE, HPL_UPDATE_DEPTH, HPL_BULK_SLICES, HPL_F64_DEPTH).
+The schedule, update depth, slice count and tail are read when a context is
+created (HPL_SCHEDULE, HPL_UPDATE_DEPTH, HPL_BULK_SLICES, HPL_F64_DEPTH,
+HPL_TAIL_M, HPL_TAIL_DEPTH).
 """
 import os
 
@@ -22,7 +23,8 @@ from hpcc_fpga_amd import Context
 
 pytestmark = pytest.mark.gpu
 
-KEYS = ("HPL_SCHEDULE", "HPL_UPDATE_DEPTH", "HPL_BULK_SLICES", "HPL_F64_DEPTH")
+KEYS = ("HPL_SCHEDULE", "HPL_UPDATE_DEPTH", "HPL_BULK_SLICES", "HPL_F64_DEPTH", "HPL_TAIL_M",
+        "HPL_TAIL_DEPTH")
 
 
 def _context(env):
@@ -52,17 +54,24 @@ def _factor(env, n, B, prec="fp32", seed=5):
         c.close()
 
 
-@pytest.mark.parametrize("n,B,depth,slices", [
-    (2048, 128, 3, 0),   # 16 blocks: groups of 3, the last one short
-    (1792, 256, 3, 0),   # 7 blocks
-    (2560, 128, 4, 3),   # fewer slices than steps per group
-    (1024, 64, 2, 5),    # more slices than steps per group
-    (4096, 256, 5, 0),
+@pytest.mark.parametrize("n,B,depth,slices,tail", [
+    (2048, 128, 3, 0, None),     # 16 blocks: groups of 3, the last one short
+    (1792, 256, 3, 0, None),     # 7 blocks
+    (2560, 128, 4, 3, None),     # fewer slices than steps per group
+    (1024, 64, 2, 5, None),      # more slices than steps per group
+    (4096, 256, 5, 0, None),
+    (4096, 256, 3, 0, (0, 0)),     # no tail
+    (4096, 256, 3, 0, (1536, 1)),  # groups of 3, then single steps
+    (3072, 128, 4, 0, (1280, 2)),  # a group straddles the tail threshold
 ])
-def test_pipelined_equals_grouped_fp32(n, B, depth, slices):
+def test_pipelined_equals_grouped_fp32(n, B, depth, slices, tail):
     g = _factor({"HPL_SCHEDULE": "grouped", "HPL_UPDATE_DEPTH": str(depth)}, n, B)
-    p = _factor({"HPL_SCHEDULE": "pipelined", "HPL_UPDATE_DEPTH": str(depth),
-                 "HPL_BULK_SLICES": str(slices)}, n, B)
+    env = {"HPL_SCHEDULE": "pipelined", "HPL_UPDATE_DEPTH": str(depth),
+           "HPL_BULK_SLICES": str(slices)}
+    # None: the default tail (half the order, half the depth)
+    if tail:
+        env.update({"HPL_TAIL_M": str(tail[0]), "HPL_TAIL_DEPTH": str(tail[1])})
+    p = _factor(env, n, B)
     assert np.array_equal(g.view(np.uint32), p.view(np.uint32))
 
 
@@ -78,7 +87,9 @@ def test_pipelined_equals_grouped_fp64():
     n = 1600  # 25 blocks of 64: groups of 6, the last one short
     out = []
     for env in ({"HPL_SCHEDULE": "grouped", "HPL_F64_DEPTH": "6"},
-                {"HPL_SCHEDULE": "pipelined", "HPL_F64_DEPTH": "6"}):
+                {"HPL_SCHEDULE": "pipelined", "HPL_F64_DEPTH": "6"},
+                {"HPL_SCHEDULE": "pipelined", "HPL_F64_DEPTH": "6", "HPL_TAIL_M": "640",
+                 "HPL_TAIL_DEPTH": "2"}):
         print(f"schedule {env} FP64 n={n}", flush=True)
         c = _context(env)
         try:
@@ -89,3 +100,4 @@ def test_pipelined_equals_grouped_fp64():
         finally:
             c.close()
     assert np.array_equal(out[0].view(np.uint64), out[1].view(np.uint64))
+    assert np.array_equal(out[0].view(np.uint64), out[2].view(np.uint64))
