@@ -296,7 +296,8 @@ hpl_status panel(hpl_ctx* c, hipStream_t st, int64_t k, uint32_t B) {
     return r ? r : panel_strips(c, st, k, B);
 }
 
-// Blocked right-looking schedule with depth-1 look-ahead (the pattern of the
+// Grouped schedule (HPL_SCHEDULE=grouped; also the one used without a linear
+// strip buffer): blocked right-looking LU with depth-1 look-ahead (the pattern of the
 // reference's XRT backend, execution_xrt_pcie.hpp:328-415) and a delayed bulk
 // update.  At step k the row and column blocks of the next panel receive all
 // pending updates; the panel of step k+1 (diagonal block and strip solves)

@@ -79,10 +79,10 @@ struct hpl_ctx {
     bool binv_ready = false;    // dBInv holds the inverses of the current factor
     float* dInner = nullptr;    // recursion scratch for blocks > 128
     size_t capInner = 0;
-    void* dLpT = nullptr;       // low-precision k-contiguous top-strip copies, 2*depth slots of n x B
+    void* dLpT = nullptr;       // low-precision k-contiguous top-strip copies, slots of n x B (as dLt)
     void* dLpL = nullptr;       // low-precision k-contiguous left-strip copies, same layout
     size_t capLpT = 0, capLpL = 0;
-    float* dScT = nullptr;      // FP8 block scales of the copies, 2*depth slots of n/32
+    float* dScT = nullptr;      // FP8 block scales of the copies, slots of n/32 (as dLt)
     float* dScL = nullptr;
     size_t capScT = 0, capScL = 0;
     int prec = 0;               // trailing-update precision of the current factorisation

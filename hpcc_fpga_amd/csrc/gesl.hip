@@ -311,6 +311,100 @@ hipError_t gesl_sweeps(const float* A, int64_t lda, int64_t n, int B, const floa
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Split sweeps (one right-hand side, full diagonal-block inverses): per block
+// step a wide launch applies the solved block to every entry still to solve
+// -- the next diagonal block included, spread over workgroups of CH entries
+// -- and then one 1024-thread workgroup applies the next block's inverse, each
+// thread streaming its rows in a single batch of loads.  The fused sweep
+// kernels above leave workgroup 0 four dependent GEMV passes per step (the
+// next block's update, then its inverse, each over two passes of 256
+// threads); this puts two short launches on the critical path instead.
+// ---------------------------------------------------------------------------
+constexpr int NTD = 1024;
+
+// x_e += SIGN sum_p A[qoff + p][e] X[qoff + p] for e in [e_begin, e_end)
+template <int SIGN>
+__global__ __launch_bounds__(NT) void gesl_apply1_kernel(const float* __restrict__ A, int64_t lda,
+                                                        int B, int64_t qoff, int64_t e_begin,
+                                                        int64_t e_end, float* __restrict__ X) {
+    constexpr int CH = gesl_cfg<1>::CH;
+    __shared__ float vs[512];
+    __shared__ float red[1024];
+    const int t = threadIdx.x;
+    for (int i = t; i < B; i += NT) vs[i] = X[qoff + i];
+    __syncthreads();
+    block_gemv<1, SIGN>(A, lda, qoff, e_begin + (int64_t)blockIdx.x * CH, CH / 4, B, vs, 0, 1,
+                        red, e_end, X, 0);
+}
+
+// X[Qoff + i] := SIGN sum_p W[p][i] X[Qoff + p], W the B x B inverse (row-major)
+template <int SIGN>
+__global__ __launch_bounds__(NTD) void gesl_diag1_kernel(const float* __restrict__ W, int B,
+                                                        int64_t Qoff, float* __restrict__ X) {
+    __shared__ float z[512];
+    __shared__ float red[4096];
+    const int t = threadIdx.x;
+    for (int i = t; i < B; i += NTD) z[i] = X[Qoff + i];
+    __syncthreads();
+    const int cols = B / 4, groups = NTD / cols;  // float4 columns; row groups
+    const int c = t % cols, g = t / cols;
+    // rows g, g + groups, ..: at most B / groups = B^2 / 4096 <= 16 (B <= 256),
+    // all loads in flight before the first is used
+    f32x4 w[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const int p = g + i * groups;
+        w[i] = p < B ? *reinterpret_cast<const f32x4*>(W + (int64_t)p * B + 4 * c)
+                     : f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    }
+    f32x4 a = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const int p = g + i * groups;
+        if (p < B) {
+            const float zp = z[p];
+            a[0] = fmaf(w[i][0], zp, a[0]);
+            a[1] = fmaf(w[i][1], zp, a[1]);
+            a[2] = fmaf(w[i][2], zp, a[2]);
+            a[3] = fmaf(w[i][3], zp, a[3]);
+        }
+    }
+    *reinterpret_cast<f32x4*>(&red[g * B + 4 * c]) = a;  // groups * B = 4096
+    __syncthreads();
+    if (t < B) {
+        float s = 0.0f;
+        for (int gg = 0; gg < groups; ++gg) s += red[gg * B + t];
+        X[Qoff + t] = SIGN * s;
+    }
+}
+
+hipError_t gesl_sweeps_split(const float* A, int64_t lda, int64_t n, int B, const float* binv,
+                             float* X, hipStream_t s) {
+    constexpr int CH = gesl_cfg<1>::CH;
+    const int N = (int)(n / B);
+    const int64_t BB = (int64_t)B * B;
+    // forward: block 0, then per q the update by block q and the solve of q+1
+    gesl_diag1_kernel<1><<<1, NTD, 0, s>>>(binv, B, 0, X);
+    for (int q = 0; q + 1 < N; ++q) {
+        const int64_t e0 = (int64_t)(q + 1) * B;
+        gesl_apply1_kernel<1><<<(int)((n - e0 + CH - 1) / CH), NT, 0, s>>>(A, lda, B,
+                                                                          (int64_t)q * B, e0, n, X);
+        gesl_diag1_kernel<1><<<1, NTD, 0, s>>>(binv + (int64_t)(q + 1) * 2 * BB, B, e0, X);
+    }
+    // backward: block N-1, then per q the update of the entries before block q
+    // and the solve of q-1 (the stored U inverses are negated)
+    gesl_diag1_kernel<-1><<<1, NTD, 0, s>>>(binv + (int64_t)(N - 1) * 2 * BB + BB, B,
+                                            (int64_t)(N - 1) * B, X);
+    for (int q = N - 1; q >= 1; --q) {
+        const int64_t e1 = (int64_t)q * B;
+        gesl_apply1_kernel<-1><<<(int)((e1 + CH - 1) / CH), NT, 0, s>>>(A, lda, B, e1, 0, e1, X);
+        gesl_diag1_kernel<-1><<<1, NTD, 0, s>>>(binv + (int64_t)(q - 1) * 2 * BB + BB, B,
+                                                (int64_t)(q - 1) * B, X);
+    }
+    return hipGetLastError();
+}
+
 }  // namespace
 
 hipError_t launch_gesl_multi(const float* A, int64_t lda, int64_t n, int B, const float* inv,
@@ -322,7 +416,9 @@ hipError_t launch_gesl_multi(const float* A, int64_t lda, int64_t n, int B, cons
         const int nr = std::min(16, nrhs - r0);
         float* Xr = X + (int64_t)r0 * ldx;
         hipError_t e;
-        if (nr == 1)
+        if (nr == 1 && binv && B <= 256)
+            e = gesl_sweeps_split(A, lda, n, B, binv, Xr, s);
+        else if (nr == 1)
             e = gesl_sweeps<1>(A, lda, n, B, inv, inv_stride, binv, Xr, ldx, 1, s);
         else if (nr <= 4)
             e = gesl_sweeps<4>(A, lda, n, B, inv, inv_stride, binv, Xr, ldx, nr, s);
