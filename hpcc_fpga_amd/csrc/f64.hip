@@ -230,14 +230,27 @@ __global__ __launch_bounds__(UNT, 2) void update_f64_kernel(double* __restrict__
 // ---------------------------------------------------------------- substitution
 // forward, block q (64 entries from k0): x_i += x_k * a[k][i] (k < i, inside
 // the block), the block staged in LDS, one wave; then x_i += sum_{k in block}
+
+// The 64 x 64 diagonal block at (k0, k0) into S, one column per lane, with
+// all 64 loads of a lane in flight before the first is stored.
+__device__ __forceinline__ void load_block64(const double* __restrict__ A, int64_t lda, int64_t k0,
+                                             double (*S)[FB + 1]) {
+    const int l = threadIdx.x;
+    double v[FB];
+#pragma unroll
+    for (int r = 0; r < FB; ++r) v[r] = A[(k0 + r) * lda + k0 + l];
+#pragma unroll
+    for (int r = 0; r < FB; ++r) S[r][l] = v[r];
+    __syncthreads();
+}
+
 // x_k * a[k][i] for i >= k0 + 64.
 __global__ __launch_bounds__(64) void gesl_diag_fwd_f64(const double* __restrict__ A, int64_t lda,
                                                         int64_t k0, double* __restrict__ x) {
 #pragma clang fp contract(off)
     __shared__ double S[FB][FB + 1];
     const int l = threadIdx.x;
-    for (int r = 0; r < FB; ++r) S[r][l] = A[(k0 + r) * lda + k0 + l];
-    __syncthreads();
+    load_block64(A, lda, k0, S);
     double xi = x[k0 + l];
     for (int k = 0; k < FB - 1; ++k) {
         const double xk = bcast64(xi, k);
@@ -253,8 +266,7 @@ __global__ __launch_bounds__(64) void gesl_diag_bwd_f64(const double* __restrict
 #pragma clang fp contract(off)
     __shared__ double S[FB][FB + 1];
     const int l = threadIdx.x;
-    for (int r = 0; r < FB; ++r) S[r][l] = A[(k0 + r) * lda + k0 + l];
-    __syncthreads();
+    load_block64(A, lda, k0, S);
     double xi = x[k0 + l];
     for (int k = FB - 1; k >= 0; --k) {
         const double s = bcast64(xi, k) * S[k][k];
@@ -277,11 +289,15 @@ __global__ __launch_bounds__(256) void gesl_gemv_f64(const double* __restrict__ 
     __syncthreads();
     const int64_t i = i0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= i1) return;
+    // the block's 64 factor entries of column i, all in flight at once
+    double a[FB];
+#pragma unroll
+    for (int u = 0; u < FB; ++u) a[u] = A[(k0 + (BWD ? FB - 1 - u : u)) * lda + i];
     double acc = x[i];
-#pragma unroll 16
+#pragma unroll
     for (int u = 0; u < FB; ++u) {
         const int k = BWD ? FB - 1 - u : u;
-        acc = acc + xs[k] * A[(k0 + k) * lda + i];
+        acc = acc + xs[k] * a[u];
     }
     x[i] = acc;
 }
