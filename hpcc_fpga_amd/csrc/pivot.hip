@@ -135,9 +135,32 @@ __global__ __launch_bounds__(PNT) void panel_pivot_kernel(
     const int64_t p0 = k0 + (int64_t)g * R;  // position of local row 0
     const int rows = (int)min<int64_t>((int64_t)R, n - p0);
 
-    for (int e = t; e < rows * B; e += PNT) {
-        const int r = e / B, c = e % B;
-        tile[r * LDT + c] = X[(p0 + r) * ldx + k0 + c];
+    // the workgroup's rows of the panel as float4s, 16 per thread in flight at
+    // a time (ldx and k0 are multiples of 4, so the rows are 16-byte aligned)
+    {
+        constexpr int Q = B / 4;
+        const int nv = rows * Q;
+        for (int e0 = t; e0 < nv; e0 += 16 * PNT) {
+            float4 v[16];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
+                const int e = e0 + u * PNT;
+                if (e < nv)
+                    v[u] = *reinterpret_cast<const float4*>(X + (p0 + e / Q) * ldx + k0 +
+                                                            (e % Q) * 4);
+            }
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
+                const int e = e0 + u * PNT;
+                if (e < nv) {
+                    float* d = tile + (e / Q) * LDT + (e % Q) * 4;
+                    d[0] = v[u].x;
+                    d[1] = v[u].y;
+                    d[2] = v[u].z;
+                    d[3] = v[u].w;
+                }
+            }
+        }
     }
     for (int r = t; r < rows; r += PNT) tile[r * LDT + B] = __int_as_float((int)(p0 + r));
     __syncthreads();
@@ -418,6 +441,8 @@ hipError_t launch_panel_pivot(float* X, int64_t ldx, int64_t n, int64_t k0, int 
     const int64_t m = n - k0;
     const int G = panel_pivot_groups(m, B);
     if (G < 1 || G > max_groups) return hipErrorInvalidConfiguration;
+    // 16-byte row vectors
+    if ((ldx & 3) || (k0 & 3) || (reinterpret_cast<uintptr_t>(X) & 15)) return hipErrorInvalidValue;
     panel_reset_kernel<<<1, 256, 0, s>>>(keys, B, ctr);
     const size_t lds = panel_pivot_lds(B);
 #define HPL_PANEL(BB)                                                                           \
