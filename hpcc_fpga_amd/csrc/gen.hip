@@ -75,7 +75,20 @@ __global__ void resid_partial_kernel(const float* __restrict__ A, int64_t lda, i
     if (i >= n) return;
     const int64_t j0 = c * RCH, j1 = min(n, j0 + RCH);
     double s = 0.0, sa = 0.0;
-    for (int64_t j = j0; j < j1; ++j) {
+    int64_t j = j0;
+    // 16 rows' loads in flight at a time, then the sums in row order
+    for (; j + 16 <= j1; j += 16) {
+        float av[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) av[u] = A[(j + u) * lda + i];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const double a = (double)av[u];
+            s = fma(a, (double)x[j + u], s);
+            sa += fabs(a);
+        }
+    }
+    for (; j < j1; ++j) {
         const double a = (double)A[j * lda + i];
         s = fma(a, (double)x[j], s);
         sa += fabs(a);
@@ -219,7 +232,22 @@ __global__ void resid_partial_multi_kernel(const float* __restrict__ A, int64_t 
 #pragma unroll
     for (int r = 0; r < kMaxMultiRhs; ++r) s[r] = 0.0;
     double sa = 0.0;
-    for (int64_t j = j0; j < j1; ++j) {
+    int64_t j = j0;
+    // 16 rows' loads in flight at a time, then the sums in row order
+    for (; j + 16 <= j1; j += 16) {
+        float av[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) av[u] = A[(j + u) * lda + i];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const double a = (double)av[u];
+#pragma unroll
+            for (int r = 0; r < kMaxMultiRhs; ++r)
+                if (r < nr) s[r] = fma(a, X[r * ldx + j + u], s[r]);
+            sa += fabs(a);
+        }
+    }
+    for (; j < j1; ++j) {
         const double a = (double)A[j * lda + i];
 #pragma unroll
         for (int r = 0; r < kMaxMultiRhs; ++r)
