@@ -344,10 +344,12 @@ hpl_status hpl_serve_release(hpl_server* server, uint64_t factor);
 hpl_status hpl_serve_stats_get(hpl_server* server, hpl_serve_stats* stats);
 
 /* ---- kernel timing for roofline accounting -------------------------------
+ * No reference counterpart (measurement only; the reference times whole
+ * GEFA/GESL calls, linpack_benchmark.hpp:306-324).
  * When enabled, every trailing-update GEMM launch is bracketed by HIP events
  * on the stream it runs on.  hpl_kernel_stats returns the number of launches
  * since the last reset, their summed duration (s) and their algorithmic
- * flops (2*m_rows*m_cols*B per launch). */
+ * flops (2*m_rows*m_cols*K per launch, K the steps applied times B). */
 hpl_status hpl_set_profiling(hpl_ctx* ctx, int enable);
 hpl_status hpl_kernel_stats(hpl_ctx* ctx, uint64_t* launches, double* seconds, double* flops);
 /* Time (s) during which at least one of those launches ran: the union of
