@@ -101,3 +101,24 @@ def test_pipelined_equals_grouped_fp64():
             c.close()
     assert np.array_equal(out[0].view(np.uint64), out[1].view(np.uint64))
     assert np.array_equal(out[0].view(np.uint64), out[2].view(np.uint64))
+
+
+def test_kernel_timing_counts_overlap():
+    """hpl_kernel_stats / hpl_kernel_busy: the union of the update launches'
+    intervals is positive and at most their summed durations; with the bulk
+    update and the panels' row/column updates on two streams it is below it."""
+    n, B = 4096, 256
+    c = _context({"HPL_SCHEDULE": "pipelined"})
+    try:
+        c.generate(n, 0, 7)
+        c.set_profiling(True)
+        c.gefa_dev(B)
+        launches, summed, flops = c.kernel_stats()
+        busy = c.kernel_busy()
+        c.set_profiling(False)
+    finally:
+        c.close()
+    print(f"launches {launches} summed {summed:.6f} s busy {busy:.6f} s", flush=True)
+    assert launches > 0 and flops > 0
+    assert 0.0 < busy <= summed * (1 + 1e-6)
+    assert busy < summed
