@@ -437,10 +437,13 @@ double elapsed_s(hpl_ctx* c) {
 
 namespace hplx {
 
-std::vector<int64_t> group_bounds(const hpl_ctx* c, int64_t N, int64_t B, int64_t depth) {
+std::vector<int64_t> group_bounds(const hpl_ctx* c, int64_t N, int64_t B, int64_t depth,
+                                  int64_t tail_cap) {
     const int64_t n = N * B;
     const int64_t tail_m = c->tail_m < 0 ? n / 2 : c->tail_m;
-    const int64_t tail_d = c->tail_depth > 0 ? c->tail_depth : std::max<int64_t>(1, depth / 2);
+    const int64_t tail_d = c->tail_depth > 0
+                               ? c->tail_depth
+                               : std::max<int64_t>(1, std::min<int64_t>(depth / 2, tail_cap));
     std::vector<int64_t> b{0};
     while (b.back() < N) {
         const bool tail = tail_m > 0 && n - b.back() * B <= tail_m;

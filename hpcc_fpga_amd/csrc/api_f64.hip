@@ -145,7 +145,8 @@ hpl_status schedule64(hpl_ctx* c) {
 // slices; the streams join at the end of the phase.
 hpl_status schedule64_pipelined(hpl_ctx* c) {
     const int64_t n = (int64_t)c->n64, B = f64_block(), N = n / B;
-    const std::vector<int64_t> bnd = hplx::group_bounds(c, N, B, f64_depth(c));
+    // tail groups of at most 3 steps
+    const std::vector<int64_t> bnd = hplx::group_bounds(c, N, B, f64_depth(c), 3);
     const int64_t G = (int64_t)bnd.size() - 1;
     hpl_status r;
     for (int64_t g = 0; g < G; ++g) {

@@ -138,7 +138,9 @@ struct hpl_ctx {
     uint64_t n64 = 0, lda64 = 0;
     // Steps per FP64 bulk update (K = depth_f64 * 64); f64_linear: one strip
     // slot per step (pipelined schedule) instead of 2 * depth_f64 reused.
-    int depth_f64 = 6;
+    // At n = 32768, depth 8 with tail groups of 3 measured 472-477 ms
+    // against 482-488 for depth 6 (tail 3), 478-479 for depth 8 (tail 4).
+    int depth_f64 = 8;
     bool f64_linear = false;
     bool have_orig64 = false;
     bool factored64 = false;
@@ -185,8 +187,10 @@ uint64_t device_lda(uint64_t n);
 // Group boundaries, in blocks, of the pipelined schedule over N blocks of B:
 // b[0] = 0 < b[1] < ... = N, `depth` blocks per group until the order left
 // is at most the tail threshold, then the tail depth (the context's tail_m and
-// tail_depth, or their defaults).
-std::vector<int64_t> group_bounds(const hpl_ctx* c, int64_t N, int64_t B, int64_t depth);
+// tail_depth, or their defaults: half the order, and half the depth but at
+// most tail_cap).
+std::vector<int64_t> group_bounds(const hpl_ctx* c, int64_t N, int64_t B, int64_t depth,
+                                  int64_t tail_cap = 16);
 
 // The full diagonal-block inverses of the resident factor (computed once per
 // factorisation, for blocks up to 256); dBInv stays null for larger blocks.

@@ -89,7 +89,8 @@ def test_pipelined_equals_grouped_fp64():
     for env in ({"HPL_SCHEDULE": "grouped", "HPL_F64_DEPTH": "6"},
                 {"HPL_SCHEDULE": "pipelined", "HPL_F64_DEPTH": "6"},
                 {"HPL_SCHEDULE": "pipelined", "HPL_F64_DEPTH": "6", "HPL_TAIL_M": "640",
-                 "HPL_TAIL_DEPTH": "2"}):
+                 "HPL_TAIL_DEPTH": "2"},
+                {"HPL_SCHEDULE": "pipelined"}):  # the default depth and tail
         print(f"schedule {env} FP64 n={n}", flush=True)
         c = _context(env)
         try:
@@ -101,6 +102,7 @@ def test_pipelined_equals_grouped_fp64():
             c.close()
     assert np.array_equal(out[0].view(np.uint64), out[1].view(np.uint64))
     assert np.array_equal(out[0].view(np.uint64), out[2].view(np.uint64))
+    assert np.array_equal(out[0].view(np.uint64), out[3].view(np.uint64))
 
 
 def test_kernel_timing_counts_overlap():
