@@ -373,10 +373,14 @@ hpl_status gefa_schedule(hpl_ctx* c, uint32_t B) {
 //
 // Groups need not be equal: hplx::group_bounds gives their boundaries, so the
 // last part of the factorisation can use shorter groups (tail_m, tail_depth).
+// FP32 defaults to D = 4 with single steps in the tail (n = 32768: GEFA
+// 203.6-204.9 ms against 204.2-207.9 for D = 3; D = 4 with tail groups of 2:
+// 204.3-205.1; D = 5 and 6 with single steps: 203.8-205.6, 204.7-205.9).
 hpl_status gefa_pipelined(hpl_ctx* c, uint32_t B) {
     const int64_t n = (int64_t)c->n;
     const int64_t N = n / B;
-    const std::vector<int64_t> bnd = hplx::group_bounds(c, N, B, c->depth);
+    const std::vector<int64_t> bnd =
+        hplx::group_bounds(c, N, B, c->depth, c->prec == HPL_PREC_FP32 ? 1 : 16);
     const int64_t G = (int64_t)bnd.size() - 1;
     hpl_status r;
     for (int64_t g = 0; g < G; ++g) {

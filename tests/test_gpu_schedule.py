@@ -60,6 +60,7 @@ def _factor(env, n, B, prec="fp32", seed=5):
     (2560, 128, 4, 3, None),     # fewer slices than steps per group
     (1024, 64, 2, 5, None),      # more slices than steps per group
     (4096, 256, 5, 0, None),
+    (4096, 256, 4, 0, None),       # the default depth, slices and tail
     (4096, 256, 3, 0, (0, 0)),     # no tail
     (4096, 256, 3, 0, (1536, 1)),  # groups of 3, then single steps
     (3072, 128, 4, 0, (1280, 2)),  # a group straddles the tail threshold
@@ -68,7 +69,7 @@ def test_pipelined_equals_grouped_fp32(n, B, depth, slices, tail):
     g = _factor({"HPL_SCHEDULE": "grouped", "HPL_UPDATE_DEPTH": str(depth)}, n, B)
     env = {"HPL_SCHEDULE": "pipelined", "HPL_UPDATE_DEPTH": str(depth),
            "HPL_BULK_SLICES": str(slices)}
-    # None: the default tail (half the order, half the depth)
+    # None: the default tail (single steps once the order left is at most n/2)
     if tail:
         env.update({"HPL_TAIL_M": str(tail[0]), "HPL_TAIL_DEPTH": str(tail[1])})
     p = _factor(env, n, B)
