@@ -423,7 +423,7 @@ def main():
 
     extra = {}
     if not args.no_extra:
-        extra["pivoted"] = pivoted_section(ctx, n, 128)
+        extra["pivoted"] = pivoted_section(ctx, n, 256)
         extra["fp64"] = fp64_section(ctx, n)
         extra["mixed_precision"] = mixed_precision_section(ctx, args.mixed_n, B)
         extra["batched"] = batched_section(ctx)

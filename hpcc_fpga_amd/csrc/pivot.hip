@@ -165,12 +165,19 @@ __global__ __launch_bounds__(PNT) void panel_pivot_kernel(
     for (int r = t; r < rows; r += PNT) tile[r * LDT + B] = __int_as_float((int)(p0 + r));
     __syncthreads();
 
-    for (int c = 0; c < B; ++c) {
+    // Column c's candidate search, the publication of the candidate row and
+    // of row k, and the arrival at column c's barrier.  With upd (every
+    // column after the first) the workgroup has applied the previous column's
+    // rank-1 update to column c only: the published rows get it on the fly
+    // for their columns beyond c (the same float product and sum), and the
+    // workgroup updates those columns after arriving, while the other
+    // workgroups arrive (a split barrier).
+    unsigned long long best = 0;
+    auto search_arrive = [&](int c, bool upd) {
         const int64_t k = k0 + c;
         const int buf = c & 1;
-        const int64_t rk = k - p0;  // local row of position k (may be outside)
-        // ---- candidate of this workgroup: rows at positions >= k
-        unsigned long long best = 0;
+        const int64_t rk = k - p0;
+        best = 0;
         for (int r = (int)max<int64_t>(0, rk) + t; r < rows; r += PNT)
             best = max(best, pivot_key(tile[r * LDT + c], p0 + r, k));
         best = wave_max_u64(best);
@@ -179,22 +186,32 @@ __global__ __launch_bounds__(PNT) void panel_pivot_kernel(
         best = wkey[0];
 #pragma unroll
         for (int w = 1; w < PNT / 64; ++w) best = max(best, wkey[w]);
+        auto cur = [&](int r, int j) {
+            float v = tile[r * LDT + j];
+            if (upd && j > c && j < B) v = v + lcol[r] * urow[j];
+            return v;
+        };
         if (best != 0) {
             float* slot = pub + ((int64_t)buf * G + g) * LDT;
             const int rl = (int)(key_position(best, k) - p0);
-            for (int j = t; j <= B; j += PNT) publish(slot + j, tile[rl * LDT + j]);
+            for (int j = t; j <= B; j += PNT) publish(slot + j, cur(rl, j));
         }
         if (rk >= 0 && rk < rows)
-            for (int j = t; j <= B; j += PNT) publish(krow + buf * LDT + j, tile[rk * LDT + j]);
+            for (int j = t; j <= B; j += PNT) publish(krow + buf * LDT + j, cur((int)rk, j));
         __builtin_amdgcn_s_waitcnt(0);  // this wave's stores have completed
         __syncthreads();
-        // ---- grid barrier
         if (t == 0) {
             if (best != 0) {
                 atomicMax(keys + c, best);
                 __builtin_amdgcn_s_waitcnt(0);
             }
             __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    };
+    // wait until all G workgroups have arrived at column c's barrier; true
+    // when the panel was aborted (a wait past the spin limit)
+    auto wait_all = [&](int c) {
+        if (t == 0) {
             const unsigned target = (unsigned)(c + 1) * (unsigned)G;
             unsigned v;
             long spins = 0;
@@ -212,7 +229,15 @@ __global__ __launch_bounds__(PNT) void panel_pivot_kernel(
             abort_flag = v >= kAbort;
         }
         __syncthreads();
-        if (abort_flag) return;
+        return abort_flag != 0;
+    };
+
+    search_arrive(0, false);
+    for (int c = 0; c < B; ++c) {
+        if (wait_all(c)) return;
+        const int64_t k = k0 + c;
+        const int buf = c & 1;
+        const int64_t rk = k - p0;  // local row of position k (may be outside)
         // ---- pivot, exchange
         const unsigned long long kk =
             __hip_atomic_load(keys + c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -241,7 +266,7 @@ __global__ __launch_bounds__(PNT) void panel_pivot_kernel(
         __syncthreads();
         // ---- multipliers of the rows below position k (gefa_ref: double
         // reciprocal and product, rounded once), then the rank-1 update as a
-        // float product and a float sum, one column per TPC threads
+        // float product and a float sum
         const int rs1 = (int)max<int64_t>(0, k + 1 - p0);
         const double inv = -1.0 / (double)piv;
         for (int r = rs1 + t; r < rows; r += PNT) {
@@ -250,8 +275,18 @@ __global__ __launch_bounds__(PNT) void panel_pivot_kernel(
             lcol[r] = l;
         }
         __syncthreads();
+        if (c + 1 == B) break;
+        // column c+1 first, one row per thread: the next search reads it
+        {
+            const float u1 = urow[c + 1];
+            for (int r = rs1 + t; r < rows; r += PNT)
+                tile[r * LDT + c + 1] = tile[r * LDT + c + 1] + lcol[r] * u1;
+        }
+        __syncthreads();
+        search_arrive(c + 1, true);
+        // the other columns, one column per TPC threads
         const int j = t % B;
-        if (j > c) {
+        if (j > c + 1) {
             const float uj = urow[j];
             int r = rs1 + t / B;
             for (; r + 7 * TPC < rows; r += 8 * TPC) {
