@@ -233,7 +233,9 @@ class ServeCore {
         if (out) *out = j->resp;
         if (j->resp.status != HPL_SUCCESS) *err = j->err;
         const hpl_status st = j->resp.status;
-        jobs_.erase(it);
+        // the wait released the lock: a submit meanwhile may have rehashed
+        // jobs_ and invalidated `it`, so erase by key
+        jobs_.erase(ticket);
         return st;
     }
 

@@ -217,9 +217,44 @@ static void check_loop() {
     }
 }
 
+// A waiter blocks while another thread submits many more requests (the job
+// table grows and rehashes during the wait); the waiter must still retire its
+// own ticket and every other ticket stays valid.
+static void check_concurrent_wait() {
+    FakeExecutor ex;
+    ServeCore core(&ex, 64000000, 2, 16, 1024, HPL_PREC_FP8);
+    std::string err;
+    const int N = 400;
+    std::vector<std::vector<double>> bs(N), xs(N);
+    std::vector<uint64_t> tickets(N);
+    hpl_request r0 = make_req(1024, 16, bs[0], xs[0]);
+    CHECK(core.submit(r0, &tickets[0], &err) == HPL_SUCCESS);
+    std::atomic<int> waited{0};
+    std::thread waiter([&] {
+        hpl_response resp{};
+        std::string e;
+        CHECK(core.wait(tickets[0], &resp, &e) == HPL_SUCCESS);
+        waited = 1;
+    });
+    for (int i = 1; i < N; ++i) {
+        hpl_request r = make_req(64 + (uint64_t)i % 200, 1, bs[i], xs[i]);
+        CHECK(core.submit(r, &tickets[i], &err) == HPL_SUCCESS);
+    }
+    waiter.join();
+    CHECK(waited.load() == 1);
+    for (int i = 1; i < N; ++i) {
+        hpl_response resp{};
+        CHECK(core.wait(tickets[i], &resp, &err) == HPL_SUCCESS);
+        for (size_t k = 0; k < bs[i].size(); ++k) CHECK(xs[i][k] == 2.0 * bs[i][k]);
+    }
+    hpl_response resp{};
+    CHECK(core.wait(tickets[0], &resp, &err) == HPL_ERR_INVALID_ARG);  // retired once
+}
+
 int main() {
     check_arena();
     check_loop();
+    check_concurrent_wait();
     std::printf("OK\n");
     return 0;
 }
