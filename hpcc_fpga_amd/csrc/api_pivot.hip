@@ -134,9 +134,10 @@ hpl_status gefa_pivot(hpl_ctx* c, uint32_t B) {
     for (int64_t k = 0; k < N; ++k) {
         const int64_t k0 = k * B, m1 = n - k0 - B;
         PCHK(c, launch_panel_pivot(c->dA, lda, n, k0, (int)B, pub, krow, keys, sync, c->dLt, n,
-                                   c->dDiagU, c->dPiv, src, c->dStatus, c->pv_max_groups, s));
+                                   c->dDiagU, c->dPiv, src, c->dStatus, c->pv_max_groups, s,
+                                   c->pv_spin_limit));
         PCHK(c, launch_row_moves(c->dA, lda, n, k0, (int)B, c->dPiv, src, list, count, Y, n,
-                                 c->dPerm, s));
+                                 c->dPerm, c->dStatus, s));
         if (m1 > 0) {
             float* inv = c->dInv + k * inv_stride;
             PCHK(c, launch_unit_lower(c->dA, lda, k0, (int)B, D, s));

@@ -124,6 +124,11 @@ struct hpl_ctx {
     float* dDiagU = nullptr;    // u_kk
     int pv_max_groups = 0;      // co-resident panel workgroups at block pv_B
     uint32_t pv_B = 0;
+    // polls of the panel's grid barrier before it gives up and fails the
+    // factorisation (0: the kernel's default, about a second); a negative
+    // value -k makes every barrier give up after k polls, which exercises
+    // that error path with every kernel still exiting normally
+    long pv_spin_limit = 0;
 
     // FP64 system (api_f64.hip): matrix, kept original, b, x, transposed
     // left strips (slots of 64 x n) and the residual partials

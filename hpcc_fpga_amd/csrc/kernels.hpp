@@ -93,16 +93,20 @@ hipError_t panel_pivot_prepare(int B, int* max_groups);
 // the diagonal), Lt[c][pos] for pos >= k0 + B, diagU/ipvt at k0.., and the row
 // of origin src[pos] of positions k0..n-1.  status |= 1: zero or non-finite
 // pivot; |= 2: the grid barrier timed out; |= 4: a pivot key outside the
-// panel was read (a broken barrier; the row is left in place).
+// panel was read (a broken barrier; the row is left in place).  spin_limit:
+// polls of the barrier before a workgroup gives up (0: about a second; < 0:
+// give up after -spin_limit polls on every barrier, the error path).
 hipError_t launch_panel_pivot(float* X, int64_t ldx, int64_t n, int64_t k0, int B, float* pub,
                               float* krow, unsigned long long* keys, unsigned* ctr, float* Lt,
                               int64_t ldl, float* diagU, int* ipvt, int* src, int* status,
-                              int max_groups, hipStream_t s);
+                              int max_groups, hipStream_t s, long spin_limit = 0);
 // apply panel k0's row moves to the columns outside it and to perm; list:
-// 4 B ints, count: one int, Y: 2 B x ldy floats
+// 4 B ints, count: one int, Y: 2 B x ldy floats.  Nothing moves when status
+// carries a panel failure (bits 2, 4) or a listed position lies outside
+// [k0, n) (then status |= 4).
 hipError_t launch_row_moves(float* X, int64_t ldx, int64_t n, int64_t k0, int B, const int* ipvt,
                             const int* src, int* list, int* count, float* Y, int64_t ldy,
-                            int* perm, hipStream_t s);
+                            int* perm, int* status, hipStream_t s);
 hipError_t launch_iota(int* p, int64_t n, hipStream_t s);
 // unit lower triangle of X's diagonal block at k0 into D (B x B)
 hipError_t launch_unit_lower(const float* X, int64_t ldx, int64_t k0, int B, float* D,

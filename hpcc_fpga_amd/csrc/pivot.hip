@@ -42,7 +42,7 @@ namespace {
 
 constexpr int PNT = 256;                // threads of the panel kernel
 constexpr unsigned kAbort = 1u << 30;   // added to the barrier counter on abort
-constexpr long kSpinLimit = 1L << 21;   // polls (each >= ~0.5 us) before abort
+constexpr long kSpinLimit = 1L << 21;   // default polls (each >= ~0.5 us) before abort
 
 // ---------------------------------------------------------------- transpose
 // In-place transpose of an n x n matrix: workgroup (J, I), I <= J, exchanges
@@ -120,7 +120,7 @@ __global__ __launch_bounds__(PNT) void panel_pivot_kernel(
     float* __restrict__ X, int64_t ldx, int64_t n, int64_t k0, int R, float* __restrict__ pub,
     float* __restrict__ krow, unsigned long long* __restrict__ keys, unsigned* __restrict__ ctr,
     float* __restrict__ Lt, int64_t ldl, float* __restrict__ diagU, int* __restrict__ ipvt,
-    int* __restrict__ src, int* __restrict__ status) {
+    int* __restrict__ src, int* __restrict__ status, long spin_limit, unsigned phantom) {
 #pragma clang fp contract(off)
     constexpr int LDT = B + 1;
     constexpr int TPC = PNT / B;  // threads per column in the update
@@ -212,13 +212,14 @@ __global__ __launch_bounds__(PNT) void panel_pivot_kernel(
     // when the panel was aborted (a wait past the spin limit)
     auto wait_all = [&](int c) {
         if (t == 0) {
-            const unsigned target = (unsigned)(c + 1) * (unsigned)G;
+            // phantom > 0 (a test setting) waits for arrivals that never come
+            const unsigned target = (unsigned)(c + 1) * (unsigned)G + phantom;
             unsigned v;
             long spins = 0;
             while ((v = __hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) <
                    target) {
                 __builtin_amdgcn_s_sleep(1);
-                if (++spins > kSpinLimit) {
+                if (++spins > spin_limit) {
                     atomicOr(status, 2);
                     v = __hip_atomic_fetch_add(ctr, kAbort, __ATOMIC_RELAXED,
                                                __HIP_MEMORY_SCOPE_AGENT) +
@@ -327,34 +328,65 @@ __global__ void panel_reset_kernel(unsigned long long* keys, int B, unsigned* ct
 // The row moves of panel k0: every position t in [k0, k0+B) and every pivot
 // position beyond it (once) whose row of origin src[t] differs from t.
 // list[2i] = t, list[2i+1] = src[t]; *count = number of moves.  One workgroup.
+// The moves index rows by the panel's outputs (ipvt, src), so they are taken
+// only from a panel that completed: if the panel's barrier gave up (status
+// bits 2 and 4) ipvt and src may be stale, and no move is listed.  Every
+// position read or listed is checked against [k0, n) as well; one outside it
+// raises status bit 4 and empties the list.
 __global__ __launch_bounds__(256) void move_list_kernel(const int* __restrict__ ipvt,
                                                         const int* __restrict__ src, int64_t k0,
-                                                        int B, int* __restrict__ list,
-                                                        int* __restrict__ count) {
+                                                        int64_t n, int B, int* __restrict__ list,
+                                                        int* __restrict__ count,
+                                                        int* __restrict__ status) {
     __shared__ int piv[256];
-    __shared__ int cnt;
+    __shared__ int cnt, bad;
     const int c = threadIdx.x;
-    if (c == 0) cnt = 0;
+    // the panel kernel ran before on this stream: status is final here and
+    // the same for every thread
+    if (*status & 6) {
+        if (c == 0) *count = 0;
+        return;
+    }
+    if (c == 0) {
+        cnt = 0;
+        bad = 0;
+    }
+    auto inside = [&](int64_t v) { return v >= k0 && v < n; };
     if (c < B) piv[c] = ipvt[k0 + c];
     __syncthreads();
     if (c < B) {
         const int t1 = (int)(k0 + c);
-        if (src[t1] != t1) {
+        const int s1 = src[t1];
+        if (!inside(s1)) {
+            bad = 1;
+        } else if (s1 != t1) {
             const int i = atomicAdd(&cnt, 1);
             list[2 * i] = t1;
-            list[2 * i + 1] = src[t1];
+            list[2 * i + 1] = s1;
         }
         const int t2 = piv[c];
-        bool first = t2 >= k0 + B;
-        for (int q = 0; q < c && first; ++q) first = piv[q] != t2;
-        if (first && src[t2] != t2) {
-            const int i = atomicAdd(&cnt, 1);
-            list[2 * i] = t2;
-            list[2 * i + 1] = src[t2];
+        if (t2 < t1 || t2 >= n) {
+            bad = 1;
+        } else {
+            bool first = t2 >= k0 + B;
+            for (int q = 0; q < c && first; ++q) first = piv[q] != t2;
+            if (first) {
+                const int s2 = src[t2];
+                if (!inside(s2)) {
+                    bad = 1;
+                } else if (s2 != t2) {
+                    const int i = atomicAdd(&cnt, 1);
+                    list[2 * i] = t2;
+                    list[2 * i + 1] = s2;
+                }
+            }
         }
     }
     __syncthreads();
-    if (c == 0) *count = cnt;
+    if (c == 0) {
+        *count = bad ? 0 : cnt;
+        if (bad) atomicOr(status, 4);
+    }
 }
 
 // Y[i][col] = X[src_i][col] over the columns outside [k0, k0+B) (4 per thread)
@@ -471,8 +503,13 @@ hipError_t panel_pivot_prepare(int B, int* max_groups) {
 hipError_t launch_panel_pivot(float* X, int64_t ldx, int64_t n, int64_t k0, int B, float* pub,
                               float* krow, unsigned long long* keys, unsigned* ctr, float* Lt,
                               int64_t ldl, float* diagU, int* ipvt, int* src, int* status,
-                              int max_groups, hipStream_t s) {
+                              int max_groups, hipStream_t s, long spin_limit) {
     const int R = panel_pivot_rows(B);
+    // spin_limit < 0: give up after -spin_limit polls, waiting for one more
+    // workgroup than the grid holds, so that every barrier gives up (the
+    // error path, exercised from the host with every kernel exiting)
+    const unsigned phantom = spin_limit < 0 ? 1u : 0u;
+    spin_limit = spin_limit == 0 ? kSpinLimit : (spin_limit < 0 ? -spin_limit : spin_limit);
     const int64_t m = n - k0;
     const int G = panel_pivot_groups(m, B);
     if (G < 1 || G > max_groups) return hipErrorInvalidConfiguration;
@@ -483,7 +520,8 @@ hipError_t launch_panel_pivot(float* X, int64_t ldx, int64_t n, int64_t k0, int 
 #define HPL_PANEL(BB)                                                                           \
     case BB:                                                                                    \
         panel_pivot_kernel<BB><<<G, PNT, lds, s>>>(X, ldx, n, k0, R, pub, krow, keys, ctr, Lt, \
-                                                   ldl, diagU, ipvt, src, status);              \
+                                                   ldl, diagU, ipvt, src, status, spin_limit,   \
+                                                   phantom);                                    \
         break;
     switch (B) {
         HPL_PANEL(32)
@@ -498,8 +536,9 @@ hipError_t launch_panel_pivot(float* X, int64_t ldx, int64_t n, int64_t k0, int 
 
 hipError_t launch_row_moves(float* X, int64_t ldx, int64_t n, int64_t k0, int B, const int* ipvt,
                             const int* src, int* list, int* count, float* Y, int64_t ldy,
-                            int* perm, hipStream_t s) {
-    move_list_kernel<<<1, 256, 0, s>>>(ipvt, src, k0, B, list, count);
+                            int* perm, int* status, hipStream_t s) {
+    if (B < 1 || B > 256 || !status) return hipErrorInvalidValue;
+    move_list_kernel<<<1, 256, 0, s>>>(ipvt, src, k0, n, B, list, count, status);
     dim3 g((unsigned)((n + 1023) / 1024), (unsigned)(2 * B));
     move_gather_kernel<<<g, 256, 0, s>>>(X, ldx, n, k0, B, list, count, Y, ldy);
     move_scatter_kernel<<<g, 256, 0, s>>>(X, ldx, n, k0, B, list, count, Y, ldy);

@@ -197,3 +197,29 @@ def test_pivot_errors(ctx):
     finally:
         c.close()
     del ipvt
+
+
+def test_panel_barrier_gives_up_cleanly(ctx, monkeypatch):
+    """The panel's bounded grid-barrier wait is forced to give up from the host
+    (HPL_PANEL_SPIN_LIMIT=-4: every barrier waits for one workgroup more than
+    the grid holds and gives up after 4 polls).  The factorisation must fail
+    with HPL_ERR_HIP, no row move may use the unfinished panel's pivots, every
+    kernel must exit, and the device must stay usable: a normal pivoted
+    factorisation afterwards matches gefa_ref's pivots."""
+    n, B = 1024, 128
+    _say(f"pivot barrier abort n={n} B={B}")
+    A, _, _, _ = O.generate_ref(n, seed=5, diag_dominant=False)
+    monkeypatch.setenv("HPL_PANEL_SPIN_LIMIT", "-4")
+    c = Context(0)
+    monkeypatch.delenv("HPL_PANEL_SPIN_LIMIT")
+    try:
+        with pytest.raises(HplError) as e:
+            c.gefa_pvt(A.copy(), B)
+        assert e.value.status == 2, e.value
+        assert "barrier" in e.value.message
+    finally:
+        c.close()
+    F = A.copy()
+    ipvt, _ = ctx.gefa_pvt(F, B)
+    _, ipvt_ref = O.gefa_pvt(A)
+    assert np.array_equal(ipvt, ipvt_ref)
