@@ -85,9 +85,12 @@ const char* hpl_version(void);
  * (LINPACK/src/host/execution_types/execution_pcie.hpp:53-728).
  * hpl_gefa uploads a, factorises it on the device and writes the factor back
  * into a (in place, dgefa encoding), as calculate() leaves data.A
- * (execution_pcie.hpp:695-703).  block must be a power of two in [32, 256]
- * dividing n.  hpl_gesl solves M x = b with a factor produced by hpl_gefa
- * (host buffer lu) and overwrites b with x, as gesl_ref_nopvt does
+ * (execution_pcie.hpp:695-703).  block must be a power of two in [32, 512]
+ * dividing n (blocks above 128 factor their diagonal block by a 2 x 2
+ * recursion; the pivoted calls below take blocks up to 256, and the
+ * substitution applies whole diagonal-block inverses only up to 256, above
+ * it the 32 x 32 sequential solves).  hpl_gesl solves M x = b with a factor
+ * produced by hpl_gefa (host buffer lu) and overwrites b with x, as gesl_ref_nopvt does
  * (LINPACK/src/host/linpack_data.cpp:199-229). */
 hpl_status hpl_gefa(hpl_ctx* ctx, float* a, uint64_t n, uint64_t lda, uint32_t block,
                     hpl_timing* timing);

@@ -8,6 +8,10 @@ Each oracle function is checked against something other than itself:
                       and the invariants of test_host_reference_implementations.cpp:30-53;
   generate_sized    - independent numpy splitmix64 restatement (bit-exact),
                       dominance margin, b = M 1 in FP64;
+  generate_batch_system, batch_orders
+                    - the same splitmix64 restatement, identity pad, entry
+                      ranges, b = M 1 (x = 1 in an FP64 solve; b = A 1 fails);
+  gefa_nopvt_omp    - bit-identical to gefa_nopvt for 1, 3 and 8 threads;
   gefa_nopvt        - decoded factor reproduces M (numpy FP64), equals scipy's
                       LU (no row exchange happens for these column-dominant
                       matrices), brute force on 2x2/3x3, committed golden vector;
@@ -133,6 +137,55 @@ def test_sized_generator_mode1():
     assert np.array_equal(b, b64.astype(np.float32))
     xs = _splitmix_uniform(7, 1, np.arange(n)).astype(np.float32)
     assert np.array_equal(xt, (np.float32(2) * xs - np.float32(1)).astype(np.float64))
+
+
+def test_batch_system_generator():
+    """orc_generate_batch_system against an independent splitmix64
+    restatement (stream 2 + s of the seed, entries U[-0.5,0.5) with n added on
+    the diagonal), the identity pad and a zero right-hand side there, and the
+    orientation of b: b = M 1 with M = A^T, so the FP64 solve gives x = 1
+    (b = A 1 would not: A is not symmetric)."""
+    seed = 7
+    for s, n in [(0, 33), (5, 64), (11, 100)]:
+        A, b, npad = O.generate_batch_system(n, seed, s)
+        assert npad == (n + 31) // 32 * 32 and A.shape == (npad, npad)
+        U = _splitmix_uniform(seed, 2 + s, np.arange(n * n)).reshape(n, n) - np.float32(0.5)
+        U[np.arange(n), np.arange(n)] += np.float32(n)
+        assert np.array_equal(A[:n, :n].view(np.uint32), U.view(np.uint32))
+        off = A[:n, :n].copy()
+        off[np.arange(n), np.arange(n)] = 0
+        assert np.all((off >= -0.5) & (off < 0.5))
+        assert np.all((np.diag(A)[:n] >= n - 0.5) & (np.diag(A)[:n] < n + 0.5))
+        pad = A.copy()
+        pad[:n, :n] = 0
+        assert np.array_equal(pad, np.diag(np.r_[np.zeros(n), np.ones(npad - n)]).astype(np.float32))
+        assert np.all(b[n:] == 0.0)
+        M = A.T.astype(np.float64)
+        x = np.linalg.solve(M, b)
+        assert np.max(np.abs(x[:n] - 1.0)) <= 1e-13 and np.all(x[n:] == 0.0)
+        assert np.max(np.abs(np.linalg.solve(A.astype(np.float64), b)[:n] - 1.0)) > 1e-6
+
+
+def test_batch_orders_log_uniform():
+    """batch_orders: n = round(exp(log lo + u (log hi - log lo))) with u the
+    counter-based uniform of stream 1, inside [lo, hi]."""
+    o = O.batch_orders(4096, 32, 1024, 7)
+    u = _splitmix_uniform(7, 1, np.arange(4096)).astype(np.float64)
+    assert np.array_equal(o, np.rint(np.exp(np.log(32) + u * (np.log(1024) - np.log(32)))))
+    assert o.min() >= 32 and o.max() <= 1024
+    # log-uniform: the median order is near sqrt(32 * 1024) = 181
+    assert 160 <= np.median(o) <= 200
+
+
+def test_gefa_nopvt_omp_bit_identical():
+    """The parallel restatement (the OpenMP CPU timing) spreads the independent
+    columns of each step over threads; every element receives the same
+    operations in the same order, so it equals the sequential oracle bit for
+    bit, for any thread count."""
+    A, _, _, _ = O.generate_sized(300, 3, 0)
+    F = O.gefa_nopvt(A)
+    for th in (1, 3, 8):
+        assert np.array_equal(O.gefa_nopvt_omp(A, th).view(np.uint32), F.view(np.uint32))
 
 
 # --------------------------------------------------------------------------- factor / solve
