@@ -13,6 +13,9 @@
 #include <omp.h>
 #include <stdlib.h>
 
+#include <algorithm>
+#include <chrono>
+#include <cmath>
 #include <cstdint>
 #include <limits>
 #include <random>
@@ -186,6 +189,110 @@ inline std::map<std::string, std::vector<double>> calculate(const Settings& conf
 }
 
 }  // namespace hip
+
+namespace cpu {
+
+// The CPU backend (communication type "CPU", cpu_only in
+// shared/include/communication_types.hpp:58,88, which the reference's LINPACK
+// declares but has no calculate for, linpack_benchmark.hpp:306-320): the
+// reference's own host algorithms, gefa_ref_nopvt / gesl_ref_nopvt
+// (LINPACK/src/host/linpack_data.cpp:166-229) or, for a --uniform matrix,
+// gefa_ref / gesl_ref (:72-156).  Within an elimination step the columns
+// right of the pivot are independent, so they are spread over OpenMP threads;
+// every element still receives the reference's operations in the reference's
+// order (no contraction on the default x86-64 target), so the factor and the
+// solution are the reference loops' bit for bit.
+
+inline void gefa_nopvt(real_t* a, int64_t n, int64_t lda) {
+    for (int64_t k = 0; k < n; ++k) {
+        real_t* ck = a + k * lda;
+        ck[k] = -1.0 / ck[k];
+        for (int64_t i = k + 1; i < n; ++i) ck[i] *= ck[k];
+#pragma omp parallel for schedule(static) if (n - k > 128)
+        for (int64_t j = k + 1; j < n; ++j) {
+            real_t* cj = a + j * lda;
+            const real_t s = cj[k];
+            for (int64_t i = k + 1; i < n; ++i) cj[i] += ck[i] * s;
+        }
+    }
+}
+
+inline void gesl_nopvt(const real_t* a, real_t* b, int64_t n, int64_t lda) {
+    for (int64_t k = 0; k + 1 < n; ++k)
+        for (int64_t i = k + 1; i < n; ++i) b[i] += b[k] * a[lda * k + i];
+    for (int64_t k = n - 1; k >= 0; --k) {
+        const real_t scale = b[k] * a[lda * k + k];
+        b[k] = -scale;
+        for (int64_t i = 0; i < k; ++i) b[i] += scale * a[lda * k + i];
+    }
+}
+
+// Partial pivoting: column k takes the first row of largest |M(i,k)|.
+inline void gefa_pvt(real_t* a, int64_t n, int64_t lda, int* ipvt) {
+    for (int64_t i = 0; i < n; ++i) ipvt[i] = (int)i;
+    for (int64_t k = 0; k + 1 < n; ++k) {
+        real_t* ck = a + k * lda;
+        real_t max_val = std::abs(ck[k]);
+        int64_t p = k;
+        for (int64_t i = k + 1; i < n; ++i)
+            if (max_val < std::abs(ck[i])) {
+                p = i;
+                max_val = std::abs(ck[i]);
+            }
+        for (int64_t i = k; i < n; ++i) std::swap(a[i * lda + k], a[i * lda + p]);
+        ipvt[k] = (int)p;
+        for (int64_t i = k + 1; i < n; ++i) ck[i] *= -1.0 / ck[k];
+#pragma omp parallel for schedule(static) if (n - k > 128)
+        for (int64_t j = k + 1; j < n; ++j) {
+            real_t* cj = a + j * lda;
+            const real_t s = cj[k];
+            for (int64_t i = k + 1; i < n; ++i) cj[i] += ck[i] * s;
+        }
+    }
+}
+
+inline void gesl_pvt(const real_t* a, real_t* b, const int* ipvt, int64_t n, int64_t lda) {
+    for (int64_t k = 0; k + 1 < n; ++k) {
+        if (ipvt[k] != k) std::swap(b[k], b[ipvt[k]]);
+        for (int64_t i = k + 1; i < n; ++i) b[i] += b[k] * a[lda * k + i];
+    }
+    for (int64_t k = n - 1; k >= 0; --k) {
+        b[k] = b[k] / a[lda * k + k];
+        for (int64_t i = 0; i < k; ++i) b[i] -= b[k] * a[lda * k + i];
+    }
+}
+
+// Per repetition: factorise a fresh copy of the generated matrix, then solve
+// a fresh copy of b, each timed with the host clock.  Leaves the factor in
+// data.A, the pivots in data.ipvt and the solution x in data.b, as the HIP
+// backend does.
+inline std::map<std::string, std::vector<double>> calculate(const Settings& config,
+                                                             LinpackData& data) {
+    const int64_t n = config.programSettings->matrixSize;
+    const bool pivoting = !config.programSettings->isDiagonallyDominant;
+    const std::vector<real_t> a0(data.A, data.A + n * n), b0(data.b, data.b + n);
+    std::vector<double> gefa, gesl;
+    for (unsigned rep = 0; rep < config.programSettings->numRepetitions; ++rep) {
+        std::copy(a0.begin(), a0.end(), data.A);
+        std::copy(b0.begin(), b0.end(), data.b);
+        const auto t0 = std::chrono::steady_clock::now();
+        if (pivoting)
+            gefa_pvt(data.A, n, n, data.ipvt);
+        else
+            gefa_nopvt(data.A, n, n);
+        const auto t1 = std::chrono::steady_clock::now();
+        if (pivoting)
+            gesl_pvt(data.A, data.b, data.ipvt, n, n);
+        else
+            gesl_nopvt(data.A, data.b, n, n);
+        const auto t2 = std::chrono::steady_clock::now();
+        gefa.push_back(std::chrono::duration<double>(t1 - t0).count());
+        gesl.push_back(std::chrono::duration<double>(t2 - t1).count());
+    }
+    return {{"gefa", gefa}, {"gesl", gesl}};
+}
+
+}  // namespace cpu
 }  // namespace execution
 
 class LinpackBenchmark
@@ -351,6 +458,9 @@ class LinpackBenchmark
         switch (s.communicationType) {
             case CommunicationType::hip:
                 timings = execution::hip::calculate(*executionSettings, data);
+                break;
+            case CommunicationType::cpu_only:
+                timings = execution::cpu::calculate(*executionSettings, data);
                 break;
             default:
                 throw std::runtime_error("No calculate method implemented for communication type " +

@@ -66,11 +66,44 @@ def test_setup_errors_exit_1(args, msg):
 
 
 def test_unhandled_backend_exits_1():
-    """Backends other than HIP throw in executeKernel, as the reference's
-    default case does (linpack_benchmark.hpp:319)."""
-    r = run("-f", "CPU", "-m", "2", "-b", "5", "-n", "1")
+    """Backends other than HIP and CPU throw in executeKernel, as the
+    reference's default case does (linpack_benchmark.hpp:319)."""
+    r = run("-f", "PCIE", "-m", "2", "-b", "5", "-n", "1")
     assert r.returncode == 1
-    assert "No calculate method implemented for communication type CPU" in r.stderr
+    assert "No calculate method implemented for communication type PCIE" in r.stderr
+
+
+@pytest.mark.parametrize("uniform", [False, True])
+def test_cpu_backend_is_the_reference_cpu_path(tmp_path, uniform):
+    """-f CPU (cpu_only, communication_types.hpp:58,88) runs the reference's
+    own gefa_ref_nopvt/gesl_ref_nopvt (--uniform: gefa_ref/gesl_ref) inside the
+    host, with no device: its residual max|x - 1| equals, bit for bit, that of
+    the oracle's restatement of the same loops on the same ref-generator
+    matrix (mt19937(0), linpack_benchmark.hpp:211-297)."""
+    import numpy as np
+
+    from oracle import oracle as O
+
+    n = 160
+    out = tmp_path / "c.json"
+    args = ["-f", "CPU", "-m", "5", "-b", "5", "-n", "2", "--dump-json", str(out)]
+    r = run(*(args + (["--uniform"] if uniform else [])))
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert re.search(r"Communication Type\s+CPU", r.stdout)
+    assert "Validation: SUCCESS!" in r.stdout
+    m = re.search(LINPACK_REGEX, r.stdout)
+    assert m and int(m.group("size")) == n
+    d = json.loads(out.read_text())
+    assert len(d["timings"]["gefa"]) == 2 and d["validated"] is True
+    A, b, _, normb = O.generate_ref(n, seed=0, diag_dominant=not uniform)
+    if uniform:
+        F, ipvt = O.gefa_pvt(A)
+        x = O.gesl_pvt(F, b, ipvt)
+    else:
+        x = O.gesl_nopvt(O.gefa_nopvt(A), b)
+    resid, residn = O.residn_ref(x, normb)
+    assert d["errors"]["residual"] == resid
+    assert d["errors"]["residual_norm"] == pytest.approx(residn, rel=1e-12)
 
 
 @pytest.mark.gpu
