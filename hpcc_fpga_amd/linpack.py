@@ -161,6 +161,7 @@ def load_library():
         "hpl_set_profiling": ([P, i32], i32),
         "hpl_kernel_stats": ([P, C.POINTER(u64), C.POINTER(dbl), C.POINTER(dbl)], i32),
         "hpl_kernel_busy": ([P, C.POINTER(dbl)], i32),
+        "hpl_gesl_noalloc": ([P, u64, u64, P, P], i32),
     }
     for name, (args, res) in sigs.items():
         fn = getattr(L, name)

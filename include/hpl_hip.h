@@ -360,6 +360,24 @@ hpl_status hpl_kernel_stats(hpl_ctx* ctx, uint64_t* launches, double* seconds, d
  * the summed duration; flops / busy is the device's trailing-update rate. */
 hpl_status hpl_kernel_busy(hpl_ctx* ctx, double* seconds);
 
+/* ---- substitution that allocates nothing ----------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory -- code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc -- hpl_gesl's solve on the caller's device memory, with no
+ * context: lu is a device pointer to a factor in hpl_gefa's encoding (n x n,
+ * row-major, leading dimension lda, a multiple of 4, 16-byte aligned) and x a
+ * device pointer to n floats (16-byte aligned) holding b on entry and x on
+ * return.  stream is the hipStream_t to enqueue on (NULL: the null stream);
+ * the call only enqueues work, so synchronise that stream before reading x.
+ * The block is hpl_gesl's (the largest power of two <= 256 dividing n; n must
+ * be a multiple of 32) and the result equals hpl_gesl's bit for bit.  The
+ * diagonal-block inverses, n * (384 + 8 * block) bytes, live in one fixed
+ * 40 MiB device array per GPU (held until the process ends): a larger system
+ * returns HPL_ERR_BUDGET before anything is enqueued (n <= 17152 when 256
+ * divides n).  Every call on a GPU shares that array and overwrites it, so
+ * calls that overlap on one GPU (several streams or host threads) race. */
+hpl_status hpl_gesl_noalloc(const float* lu, uint64_t n, uint64_t lda, float* x, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
