@@ -35,7 +35,7 @@ MFMA_BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 matrix peak (datasheet, no s
 
 KERNEL_NAMES = {
     "fp32": "gemm_f32::update_kernel<128,128,16,32,64,2>",
-    "bf16": "update_lp_kernel<__bf16>",
+    "bf16": "update_lpw_kernel<__bf16,256,256>",
 }
 
 
@@ -127,17 +127,19 @@ def cpu_baseline(full_n, mode, d, seed=42, sizes=(1024, 2048, 4096), n_par=16384
     }
 
 
-def _roofline(kind, launches, gemm_s, gemm_flops, busy_s, peak, traffic_file):
+def _roofline(kind, launches, gemm_s, gemm_flops, gemm_bytes, busy_s, peak, traffic_file):
     achieved = gemm_flops / gemm_s / 1e12 if gemm_s > 0 else 0.0
     device_achieved = gemm_flops / busy_s / 1e12 if busy_s > 0 else 0.0
     tr = measured_traffic(traffic_file)
+    alg_bytes = gemm_bytes / max(1, launches)
     return {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
             "frac": round(achieved / peak, 4),
             "traffic": round(tr["hbm_bytes_per_launch"]) if tr else None,
             "traffic_unit": "HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
                             "copy-calibrated, profiles/" + traffic_file + ")",
-            "algorithmic_bytes_per_launch": round(tr["algorithmic_bytes_per_launch"])
-            if tr else None,
+            "traffic_over_algorithmic": round(tr["hbm_bytes_per_launch"] / alg_bytes, 3)
+            if tr and alg_bytes > 0 else None,
+            "algorithmic_bytes_per_launch": round(alg_bytes),
             "algorithmic_flops_per_launch": round(gemm_flops / max(1, launches)),
             "kernel": KERNEL_NAMES[kind],
             "launches": launches, "avg_launch_us": round(gemm_s / max(1, launches) * 1e6, 1),
@@ -184,6 +186,7 @@ def headline(ctx, n, B, d, steps, warmup, prec="bf16"):
               file=sys.stderr, flush=True)
     wall = time.perf_counter() - wall0
     launches, gemm_s, gemm_flops = ctx.kernel_stats()
+    gemm_bytes = ctx.kernel_bytes()
     busy = ctx.kernel_busy()
     ctx.set_profiling(False)
     x = ctx.download_solution64()
@@ -197,8 +200,8 @@ def headline(ctx, n, B, d, steps, warmup, prec="bf16"):
         "validation": {"backward_error_max": max(berr), "tolerance": tol,
                        "forward_error_vs_x_true": fwd, "forward_error_bound": 1e-10,
                        "corrections": sorted(set(its)), "passed": passed},
-        "roofline": _roofline(prec, launches, gemm_s, gemm_flops, busy, MFMA_BF16_PEAK_TFLOPS,
-                              f"update_lp_bf16_n{n}_traffic.json"),
+        "roofline": _roofline(prec, launches, gemm_s, gemm_flops, gemm_bytes, busy,
+                              MFMA_BF16_PEAK_TFLOPS, f"update_lp_bf16_n{n}_traffic.json"),
     }
 
 
@@ -223,6 +226,7 @@ def fp32_section(ctx, n, B, steps, warmup):
         t_lu.append(a.gefa_s)
         t_sl.append(b.gesl_s)
     launches, gemm_s, gemm_flops = ctx.kernel_stats()
+    gemm_bytes = ctx.kernel_bytes()
     busy = ctx.kernel_busy()
     ctx.set_profiling(False)
     rn, an, xn, bn = ctx.residual()
@@ -242,7 +246,7 @@ def fp32_section(ctx, n, B, steps, warmup):
             "gesl_gbps": round(4.0 * n * n / float(np.mean(t_sl)) / 1e9, 1),
             "validation": {"residn": residn, "hpl_scaled_residual": hpl,
                            "passed": bool(residn < 1 and hpl < 1)},
-            "roofline": _roofline("fp32", launches, gemm_s, gemm_flops, busy,
+            "roofline": _roofline("fp32", launches, gemm_s, gemm_flops, gemm_bytes, busy,
                                   MFMA_FP32_PEAK_TFLOPS, f"gemm_f32_n{n}_traffic.json")}
 
 

@@ -220,6 +220,9 @@ hpl_status launch_update2(hpl_ctx* c, hipStream_t st, int64_t k0, UpdRegion a, U
     if (c->prof) {
         HIPCHK(c, hipEventRecord(ev_end, st));
         c->prof_flops += 2.0 * ((double)M * N + (double)b.M * b.N) * K;
+        const double esz = c->prec == 0 ? 4.0 : (double)lp_elem_bytes(c->prec);
+        c->prof_bytes += 8.0 * ((double)M * N + (double)b.M * b.N) +
+                         esz * K * ((double)M + N + (b.M > 0 ? (double)b.M + b.N : 0.0));
     }
     return HPL_SUCCESS;
 }
@@ -386,14 +389,17 @@ hpl_status gefa_pipelined(hpl_ctx* c, uint32_t B) {
     for (int64_t g = 0; g < G; ++g) {
         const int64_t s = bnd[g + 1] * (int64_t)B;
         if (g >= 1 && s < n) {
-            // group g-1's steps in column slices of whole 128-wide tiles
+            // group g-1's steps in column slices of whole tiles (256 wide for
+            // the 16-bit update's 256 x 256 tiles, else 128)
             const int64_t steps = bnd[g] - bnd[g - 1];
             const int64_t slices = c->bulk_slices > 0 ? c->bulk_slices : steps;
-            const int64_t w = n - s, tiles = w / 128;
+            const int64_t tw = (c->prec == HPL_PREC_BF16 || c->prec == HPL_PREC_FP16) && (n - s) % 256 == 0
+                                   ? 256 : 128;
+            const int64_t w = n - s, tiles = w / tw;
             const int64_t S = std::max<int64_t>(1, std::min(slices, tiles));
             for (int64_t i = 0; i < S; ++i) {
-                const int64_t a = s + tiles * i / S * 128;
-                const int64_t b = i + 1 == S ? n : s + tiles * (i + 1) / S * 128;
+                const int64_t a = s + tiles * i / S * tw;
+                const int64_t b = i + 1 == S ? n : s + tiles * (i + 1) / S * tw;
                 r = launch_update(c, c->s1, bnd[g - 1], s, a, (int)w, (int)(b - a),
                                   (int)(steps * B), B);
                 if (r) return r;
@@ -512,6 +518,30 @@ hpl_status ensure_block_inverses(hpl_ctx* c) {
     return HPL_SUCCESS;
 }
 
+const GeslWs* gesl_workspace(hpl_ctx* c, GeslWs* ws) {
+    // own mode only: the serving engine runs several contexts' solves at
+    // once, and their single launches could not all be co-resident
+    if (c->external || !c->binv_ready || gesl_persist_groups((int64_t)c->n, (int)c->B) <= 0)
+        return nullptr;
+    if (ensure(c, c->dGeslWs, c->capGeslWs, gesl_ws_bytes((int64_t)c->n)) != HPL_SUCCESS) {
+        c->err.clear();
+        return nullptr;  // no room: the per-step launches solve
+    }
+    *ws = gesl_ws_carve(c->dGeslWs, (int64_t)c->n);
+    return ws;
+}
+
+hpl_status gesl_check(hpl_ctx* c, const GeslWs* ws) {
+    if (!ws) return HPL_SUCCESS;
+    int h = 0;
+    HIPCHK(c, hipMemcpyAsync(&h, ws->status, sizeof(int), hipMemcpyDeviceToHost, c->s0));
+    HIPCHK(c, hipStreamSynchronize(c->s0));
+    if (h & 2)
+        return fail(c, HPL_ERR_HIP,
+                    "substitution: a bounded wait gave up (workgroups not co-resident)");
+    return HPL_SUCCESS;
+}
+
 IrPlan plan_ir(uint64_t n, int nrhs) {
     IrPlan p{};
     p.work = (size_t)residual_multi_work((int64_t)n, nrhs) * sizeof(double);
@@ -546,6 +576,8 @@ hpl_status ir_solve(hpl_ctx* c, int nrhs, const double* dB, double* dX, int max_
     if (st) return st;
     st = ensure_block_inverses(c);
     if (st) return st;
+    GeslWs wsbuf;
+    const GeslWs* ws = nrhs == 1 ? gesl_workspace(c, &wsbuf) : nullptr;
     const int B = (int)c->B;
     const int64_t inv_stride = 3 * (B / 32) * 1024;
     double sc_in[16], sc_out[16], h[64];
@@ -558,7 +590,7 @@ hpl_status ir_solve(hpl_ctx* c, int nrhs, const double* dB, double* dX, int max_
     const int* perm = c->pivoted ? c->dPerm : nullptr;
     HIPCHK(c, launch_d2f_multi(dB, n, c->dIrXs, n, n, nrhs, sc_in, c->s0, perm));
     HIPCHK(c, launch_gesl_multi(c->dA, (int64_t)c->lda, n, B, c->dInv, inv_stride, c->dIrXs, n,
-                                nrhs, c->s0, c->binv_ready ? c->dBInv : nullptr));
+                                nrhs, c->s0, c->binv_ready ? c->dBInv : nullptr, ws));
     HIPCHK(c, launch_axpy_multi(dX, n, c->dIrXs, n, n, nrhs, sc_out, 1, c->s0));
     for (int it = 0;; ++it) {
         HIPCHK(c, launch_residual_multi(c->dA0, (int64_t)c->lda, n, dX, n, dB, n, nrhs, c->dWork,
@@ -592,9 +624,11 @@ hpl_status ir_solve(hpl_ctx* c, int nrhs, const double* dB, double* dX, int max_
         // columns get scale 0 and stay unchanged)
         HIPCHK(c, launch_d2f_multi(c->dIrR, n, c->dIrXs, n, n, nrhs, sc_in, c->s0, perm));
         HIPCHK(c, launch_gesl_multi(c->dA, (int64_t)c->lda, n, B, c->dInv, inv_stride, c->dIrXs,
-                                    n, nrhs, c->s0, c->binv_ready ? c->dBInv : nullptr));
+                                    n, nrhs, c->s0, c->binv_ready ? c->dBInv : nullptr, ws));
         HIPCHK(c, launch_axpy_multi(dX, n, c->dIrXs, n, n, nrhs, sc_out, 0, c->s0));
     }
+    st = gesl_check(c, ws);
+    if (st) return st;
     for (int j = 0; j < nrhs; ++j)
         if (!res[j].converged)
             return fail(c, HPL_ERR_NOT_CONVERGED,
@@ -667,7 +701,7 @@ void hpl_destroy(hpl_ctx* c) {
                         (void*)c->dRhsX, (void*)c->dGmV, (void*)c->dGmW, (void*)c->dGmC,
                         (void*)c->dGmWork, (void*)c->dBInv, c->dPv, (void*)c->dA64,
                         (void*)c->dA064, (void*)c->dB64s, (void*)c->dX64s, (void*)c->dLt64,
-                        (void*)c->dWork64, (void*)c->dOut64})
+                        (void*)c->dWork64, (void*)c->dOut64, c->dGeslWs})
             if (p) (void)hipFree(p);
     for (auto& e : c->ev) {
         (void)hipEventDestroy(e.first);
@@ -805,12 +839,15 @@ hpl_status hpl_gesl_dev(hpl_ctx* c, hpl_timing* t) {
     else
         HIPCHK(c, hipMemcpyAsync(c->dx, c->db, c->n * sizeof(float), hipMemcpyDeviceToDevice,
                                  c->s0));
+    GeslWs wsbuf;
+    const GeslWs* ws = hplx::gesl_workspace(c, &wsbuf);
     HIPCHK(c, launch_gesl(c->dA, (int64_t)c->lda, (int64_t)c->n, (int)c->B, c->dInv,
-                          3 * (c->B / 32) * 1024, c->dx, c->s0, c->binv_ready ? c->dBInv : nullptr));
+                          3 * (c->B / 32) * 1024, c->dx, c->s0, c->binv_ready ? c->dBInv : nullptr,
+                          ws));
     HIPCHK(c, hipEventRecord(c->e1, c->s0));
     HIPCHK(c, hipStreamSynchronize(c->s0));
     if (t) t->gesl_s = elapsed_s(c);
-    return HPL_SUCCESS;
+    return hplx::gesl_check(c, ws);
 }
 
 hpl_status hpl_download(hpl_ctx* c, float* a, uint64_t lda, float* x, hpl_timing* t) {
@@ -955,8 +992,12 @@ hpl_status precond_apply(hpl_ctx* c, const double* v, double* z) {
     if (st) return st;
     HIPCHK(c, launch_d2f_multi(v, n, c->dIrXs, n, n, 1, &one, c->s0,
                                c->pivoted ? c->dPerm : nullptr));
+    GeslWs wsbuf;
+    const GeslWs* ws = hplx::gesl_workspace(c, &wsbuf);
     HIPCHK(c, launch_gesl_multi(c->dA, (int64_t)c->lda, n, B, c->dInv, 3 * (B / 32) * 1024,
-                                c->dIrXs, n, 1, c->s0, c->binv_ready ? c->dBInv : nullptr));
+                                c->dIrXs, n, 1, c->s0, c->binv_ready ? c->dBInv : nullptr, ws));
+    st = hplx::gesl_check(c, ws);
+    if (st) return st;
     HIPCHK(c, launch_axpy_multi(z, n, c->dIrXs, n, n, 1, &one, 1, c->s0));
     return HPL_SUCCESS;
 }
@@ -1183,6 +1224,13 @@ hpl_status hpl_set_profiling(hpl_ctx* c, int enable) {
     c->prof = enable != 0;
     c->ev_used = 0;
     c->prof_flops = 0.0;
+    c->prof_bytes = 0.0;
+    return HPL_SUCCESS;
+}
+
+hpl_status hpl_kernel_bytes(hpl_ctx* c, double* bytes) {
+    if (!c || !bytes) return HPL_ERR_INVALID_ARG;
+    *bytes = c->prof_bytes;
     return HPL_SUCCESS;
 }
 

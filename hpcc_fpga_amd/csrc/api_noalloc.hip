@@ -4,9 +4,11 @@
 // inverses in context buffers allocated on demand.  Callers that cannot
 // allocate device memory (code under HIP graph capture, serving loops,
 // harnesses that forbid hipMalloc) pass the factor and the right-hand side in
-// their own device memory instead; the inverses then live in one fixed device
-// array, `scratch`, of kScratchBytes per GPU.  The kernels are hpl_gesl's, in
-// the same order, so the result is hpl_gesl's bit for bit.
+// their own device memory instead; the inverses and the state of the
+// single-launch solve (slice sums, flags -- which each launch clears itself)
+// then live in one fixed device array, `scratch`, of kScratchBytes per GPU.
+// The kernels are hpl_gesl's, in the same order, so the result is hpl_gesl's
+// bit for bit.
 //
 // The array's host-side name is not a device address: the entry point looks
 // up its address once per device (hipGetSymbolAddress, with that device
@@ -62,7 +64,9 @@ extern "C" hpl_status hpl_gesl_noalloc(const float* lu, uint64_t n, uint64_t lda
     const int64_t inv_stride = 3 * (B / 32) * 1024;        // floats per block
     const uint64_t inv_floats = N * (uint64_t)inv_stride;  // n * 96 floats
     const uint64_t binv_floats = 2 * n * (uint64_t)B;      // B x B inverses of L and U
-    if ((inv_floats + binv_floats) * sizeof(float) > kScratchBytes) return HPL_ERR_BUDGET;
+    const uint64_t ws_bytes = gesl_ws_bytes((int64_t)n);  // the single-launch solve's state
+    if ((inv_floats + binv_floats) * sizeof(float) + ws_bytes > kScratchBytes)
+        return HPL_ERR_BUDGET;
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return HPL_ERR_NO_DEVICE;
     float* base = nullptr;
@@ -70,11 +74,13 @@ extern "C" hpl_status hpl_gesl_noalloc(const float* lu, uint64_t n, uint64_t lda
     if (st) return st;
     float* inv = base;
     float* binv = base + inv_floats;  // inv_floats is a multiple of 1024: 4 KiB aligned
+    // binv_floats = 2 n B, a multiple of 64: the workspace stays 256-byte aligned
+    const GeslWs ws = gesl_ws_carve(binv + binv_floats, (int64_t)n);
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (launch_prep_all(lu, (int64_t)lda, (int)B, (int)N, inv, inv_stride, s) != hipSuccess ||
         launch_block_inverse(lu, (int64_t)lda, (int64_t)n, (int)B, inv, inv_stride, binv, s) !=
             hipSuccess ||
-        launch_gesl(lu, (int64_t)lda, (int64_t)n, (int)B, inv, inv_stride, x, s, binv) !=
+        launch_gesl(lu, (int64_t)lda, (int64_t)n, (int)B, inv, inv_stride, x, s, binv, &ws) !=
             hipSuccess)
         return HPL_ERR_HIP;
     return HPL_SUCCESS;

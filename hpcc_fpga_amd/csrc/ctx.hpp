@@ -76,6 +76,8 @@ struct hpl_ctx {
     size_t capInv = 0;
     float* dBInv = nullptr;     // full inverses of the diagonal blocks (substitution), N x 2 x B x B
     size_t capBInv = 0;
+    void* dGeslWs = nullptr;    // workspace of the single-launch substitution
+    size_t capGeslWs = 0;
     bool binv_ready = false;    // dBInv holds the inverses of the current factor
     float* dInner = nullptr;    // recursion scratch for blocks > 128
     size_t capInner = 0;
@@ -155,7 +157,10 @@ struct hpl_ctx {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
     size_t ev_used = 0;
     double prof_flops = 0.0;
+    double prof_bytes = 0.0;  // algorithmic HBM bytes: C read and written once, operands once
 };
+
+struct GeslWs;  // kernels.hpp
 
 namespace hplx {
 
@@ -200,6 +205,13 @@ std::vector<int64_t> group_bounds(const hpl_ctx* c, int64_t N, int64_t B, int64_
 // The full diagonal-block inverses of the resident factor (computed once per
 // factorisation, for blocks up to 256); dBInv stays null for larger blocks.
 hpl_status ensure_block_inverses(hpl_ctx* c);
+
+// Workspace of the single-launch substitution of one right-hand side with the
+// resident factor (in *ws), or null where the per-step launches solve instead
+// (external mode, no full inverses, or a block the single launch does not
+// take).  gesl_check reads its status after the solve (synchronises s0).
+const GeslWs* gesl_workspace(hpl_ctx* c, GeslWs* ws);
+hpl_status gesl_check(hpl_ctx* c, const GeslWs* ws);
 
 // Device buffer of at least `bytes` (grown with hipMalloc in the context's own
 // mode, an error in external mode).

@@ -18,9 +18,12 @@
 #include "common.hpp"
 #include "kernels.hpp"
 
+#include <cstdlib>
+
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 namespace {
 
@@ -225,6 +228,208 @@ __global__ __launch_bounds__(NT, 2) void update_lp_kernel(
             }
 }
 
+// ---------------------------------------------------------------------------
+// Wide tiles for the 16-bit update: TBM x TBN workgroup tiles (256 x 256) of
+// four 128 x 128 wave tiles, one workgroup per CU.  A wave tile of 4 x 4
+// blocks reads 8 operand vectors per 16 MFMAs, half the LDS traffic per MFMA
+// of the 64 x 64 wave tiles above, and a 256-wide tile halves the operand
+// traffic from L2 per flop.  Slabs of BK k values arrive by LDS DMA
+// (global_load_lds, 16 bytes per lane, written lane-linearly) into NBUF
+// buffers, NBUF - 1 slabs ahead of the MFMAs, so the DMA latency is covered by
+// NBUF - 1 slabs of MFMA work.  Chunk c (16 bytes) of row r sits at position
+// c ^ f(r), chosen so that the 32 rows an MFMA operand read touches at one k
+// offset fall in distinct banks whether the LDS serves 8 or 16 lanes of a
+// 16-byte read per cycle: f(r) = (r & 7) ^ ((r >> 3) & 1) for 8 chunks per row,
+// ((r >> 1) ^ (r >> 2)) & 3 for 4.  The launcher sends only regions whose
+// sides are multiples of the tile, so every tile is whole.
+template <int CPR>
+__device__ __forceinline__ int swz(int r) {
+    if constexpr (CPR == 8) return (r & 7) ^ ((r >> 3) & 1);
+    else return ((r >> 1) ^ (r >> 2)) & 3;
+}
+
+// s_waitcnt immediate that waits until at most n vector memory operations of
+// this wave are outstanding (expcnt and lgkmcnt not waited for)
+constexpr int vmcnt_le(int n) { return (n & 15) | ((n >> 4) << 14) | (7 << 4) | (15 << 8); }
+
+template <typename T, int TBM, int TBN, int BK, int NBUF>
+__global__ __launch_bounds__((TBM / 128) * (TBN / 128) * 64, 1) void update_lpw_kernel(
+    float* __restrict__ C, int64_t ldc, const T* __restrict__ L, int64_t l_ld, int64_t l_slot,
+    const T* __restrict__ Tt, int64_t t_ld, int64_t t_slot, int M, int N, int K, int Bblk,
+    LpRegion r2) {
+    typedef typename lp_traits<T>::vec vec;
+    constexpr int WM = 128, WN = 128, WAVES_M = TBM / WM, WAVES = WAVES_M * (TBN / WN);
+    constexpr int FM = WM / 32, FN = WN / 32;
+    constexpr int CPR = BK / 8;      // 16-byte chunks per slab row
+    constexpr int RPI = 64 / CPR;    // rows per DMA instruction
+    static_assert(CPR == 4 || CPR == 8, "slabs of 32 or 64 k values");
+    static_assert(WAVES == 4, "the DMA row mapping assumes four waves");
+    constexpr int IA = TBM / (RPI * WAVES), IB = TBN / (RPI * WAVES);  // DMA per wave per slab
+    constexpr int KS = BK / 16;      // MFMA k-steps per slab
+    __shared__ __attribute__((aligned(16))) u32x4 As[NBUF][TBM * CPR];
+    __shared__ __attribute__((aligned(16))) u32x4 Bs[NBUF][TBN * CPR];
+
+    int tiles_m = M / TBM, tiles_n = N / TBN;
+    int bid = blockIdx.x, nblk = tiles_m * tiles_n;
+    if (bid >= nblk) {  // second region
+        bid -= nblk;
+        nblk = gridDim.x - nblk;
+        C = r2.C;
+        L = static_cast<const T*>(r2.L);
+        Tt = static_cast<const T*>(r2.T);
+        M = r2.M;
+        N = r2.N;
+        tiles_m = M / TBM;
+        tiles_n = N / TBN;
+    }
+    int tm, tn;
+    tile_coords(bid, nblk, tiles_m, tiles_n, tm, tn);
+    const int m0 = tm * TBM, n0 = tn * TBN;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave % WAVES_M, wn = wave / WAVES_M;
+    const int lr = lane & 31, lh = lane >> 5;
+
+    // DMA instruction j of this wave moves slab rows j * 4 * RPI + rl, chunk
+    // (lane % CPR) ^ f(row); f only depends on the low row bits, which j does
+    // not change, so the per-lane part of the address is one 32-bit offset
+    // and the row base is uniform.
+    const int rl = wave * RPI + lane / CPR;
+    const int ce = ((lane % CPR) ^ swz<CPR>(rl)) * 8;  // element offset in the row
+    auto dma_slab = [&](int k0, int buf) {
+        const int64_t ks = k0 / Bblk, ko = k0 % Bblk;
+        const T* lb = L + ks * l_slot + ko;
+        const T* tb = Tt + ks * t_slot + ko;
+#pragma unroll
+        for (int j = 0; j < IA; ++j) {
+            const int r0 = m0 + j * WAVES * RPI;
+            __builtin_amdgcn_global_load_lds(
+                reinterpret_cast<const unsigned*>(lb + (int64_t)r0 * l_ld + rl * (int)l_ld + ce),
+                (__attribute__((address_space(3))) void*)(&As[buf][(j * WAVES + wave) * 64]), 16,
+                0, 0);
+        }
+#pragma unroll
+        for (int j = 0; j < IB; ++j) {
+            const int r0 = n0 + j * WAVES * RPI;
+            __builtin_amdgcn_global_load_lds(
+                reinterpret_cast<const unsigned*>(tb + (int64_t)r0 * t_ld + rl * (int)t_ld + ce),
+                (__attribute__((address_space(3))) void*)(&Bs[buf][(j * WAVES + wave) * 64]), 16,
+                0, 0);
+        }
+    };
+
+    const int nk = K / BK;
+    // the first NBUF - 1 slabs go out before the loads of C
+#pragma unroll
+    for (int p = 0; p + 1 < NBUF; ++p)
+        if (p < nk) dma_slab(p * BK, p);
+
+    // the accumulators start from C (the same MFMA sequence as the 128-tile
+    // kernel, so both give the same bits): register r of block (i, j) holds
+    // row rbase + 32 i + 4 (lane / 32) + 8 (r / 4) + r % 4, column cbase +
+    // 32 j + lane % 32 -- one row pointer per (i, r), the four blocks at
+    // immediate offsets.  The row stride, read back from LDS, lives in a VGPR,
+    // so the row offsets are formed per use rather than kept as scalars.
+    const int rbase = m0 + wm * WM, cbase = n0 + wn * WN;
+    __shared__ int sh_ldc;
+    if (tid == 0) sh_ldc = (int)ldc;
+    __syncthreads();
+    const int ldc_v = sh_ldc;
+    float* cb = C + (int64_t)(rbase + 4 * lh) * ldc + cbase + lr;
+    f32x16 acc[FM][FN];
+#pragma unroll
+    for (int i = 0; i < FM; ++i) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const float* p = cb + (int64_t)(i * 32 + 8 * (r >> 2) + (r & 3)) * ldc_v;
+#pragma unroll
+            for (int j = 0; j < FN; ++j) acc[i][j][r] = p[j * 32];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+
+    u32x4 a[2][FM], b[2][FN];
+    const int sl = swz<CPR>(lr);  // f of every row this lane reads
+    auto read_ops = [&](int buf, int ks, int slot) {
+        const int c = (2 * ks + lh) ^ sl;
+#pragma unroll
+        for (int i = 0; i < FM; ++i) a[slot][i] = As[buf][(wm * WM + i * 32 + lr) * CPR + c];
+#pragma unroll
+        for (int j = 0; j < FN; ++j) b[slot][j] = Bs[buf][(wn * WN + j * 32 + lr) * CPR + c];
+    };
+    auto mfmas = [&](int slot) {
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j)
+                acc[i][j] = lp_traits<T>::mfma(__builtin_bit_cast(vec, a[slot][i]),
+                                               __builtin_bit_cast(vec, b[slot][j]), acc[i][j]);
+    };
+
+    __builtin_amdgcn_s_waitcnt(0);  // C and the first slab
+    __syncthreads();
+    read_ops(0, 0, 0);
+    for (int kc = 0; kc < nk; ++kc) {
+        const int buf = kc % NBUF;
+        // the buffer of slab kc - 1: every wave finished reading it before the
+        // barrier that ended that slab
+        if (kc + NBUF - 1 < nk) dma_slab((kc + NBUF - 1) * BK, (kc + NBUF - 1) % NBUF);
+#pragma unroll
+        for (int ks = 0; ks + 1 < KS; ++ks) {
+            read_ops(buf, ks + 1, (ks & 1) ^ 1);
+            mfmas(ks & 1);
+        }
+#pragma unroll
+        for (int ks = 0; ks + 1 < KS; ++ks) {
+            __builtin_amdgcn_sched_group_barrier(0x100, FM + FN, 0);  // DS read
+            __builtin_amdgcn_sched_group_barrier(0x008, FM * FN, 0);  // MFMA
+        }
+        if (kc + 1 < nk) {
+            // slab kc + 1 landed: at most the DMA of the slabs after it in flight
+            if (kc + NBUF - 1 < nk)
+                __builtin_amdgcn_s_waitcnt(vmcnt_le((NBUF - 2) * (IA + IB)));
+            else
+                __builtin_amdgcn_s_waitcnt(0);
+            __syncthreads();
+            read_ops((kc + 1) % NBUF, 0, KS & 1);
+        }
+        mfmas((KS - 1) & 1);
+    }
+
+#pragma unroll
+    for (int i = 0; i < FM; ++i) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            float* p = cb + (int64_t)(i * 32 + 8 * (r >> 2) + (r & 3)) * ldc_v;
+#pragma unroll
+            for (int j = 0; j < FN; ++j) p[j * 32] = acc[i][j][r];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+// The wide-tile kernel is the 16-bit update; HPL_LP_TILES=128 selects the
+// 128 x 128 one (for measurements).
+// Slab depth of the wide kernel: 64 k values in two buffers (default) or 32
+// in four (HPL_LP_SLAB=32, three slabs of DMA ahead instead of one).  At
+// M = N = 16384 the first runs at 927 / 984 / 716 TFLOP/s for K = 3072 / 6144 /
+// 1536, the second at 713 / 761 / 570: the extra barriers of the shorter slabs
+// cost more than the deeper prefetch gains.
+int lp_wide_slab() {
+    static const int k = [] {
+        const char* e = getenv("HPL_LP_SLAB");
+        return e && atoi(e) == 32 ? 32 : 64;
+    }();
+    return k;
+}
+
+bool lp_wide() {
+    static const int w = [] {
+        const char* e = getenv("HPL_LP_TILES");
+        return e && atoi(e) == 128 ? 0 : 1;
+    }();
+    return w != 0;
+}
+
 template <typename T>
 hipError_t launch_t(float* C, int64_t ldc, const void* L, int64_t l_ld, int64_t l_slot,
                     const void* Tt, int64_t t_ld, int64_t t_slot, int M, int N, int K, int Bblk,
@@ -233,6 +438,20 @@ hipError_t launch_t(float* C, int64_t ldc, const void* L, int64_t l_ld, int64_t 
     const float* scT = nullptr;
     if (!order_regions(C, L, Tt, scL, scT, M, N, r2)) return hipSuccess;
     if (K % BK || Bblk % BK || (l_ld % 8) || (t_ld % 8)) return hipErrorInvalidValue;
+    constexpr int W = 256;
+    if (lp_wide() && M % W == 0 && N % W == 0 && r2.M % W == 0 && r2.N % W == 0) {
+        const int tiles = ((M + W - 1) / W) * ((N + W - 1) / W) +
+                          (r2.M > 0 ? ((r2.M + W - 1) / W) * ((r2.N + W - 1) / W) : 0);
+        if (lp_wide_slab() == 64)
+            update_lpw_kernel<T, W, W, 64, 2><<<tiles, (W / 128) * (W / 128) * 64, 0, s>>>(
+                C, ldc, static_cast<const T*>(L), l_ld, l_slot, static_cast<const T*>(Tt), t_ld,
+                t_slot, M, N, K, Bblk, r2);
+        else
+            update_lpw_kernel<T, W, W, 32, 4><<<tiles, (W / 128) * (W / 128) * 64, 0, s>>>(
+                C, ldc, static_cast<const T*>(L), l_ld, l_slot, static_cast<const T*>(Tt), t_ld,
+                t_slot, M, N, K, Bblk, r2);
+        return hipGetLastError();
+    }
     const int tiles = region_tiles(M, N) + region_tiles(r2.M, r2.N);
     update_lp_kernel<T><<<tiles, NT, 0, s>>>(C, ldc, static_cast<const T*>(L), l_ld, l_slot,
                                             static_cast<const T*>(Tt), t_ld, t_slot, M, N, K, Bblk,

@@ -25,6 +25,7 @@
 #include "kernels.hpp"
 
 #include <algorithm>
+#include <cstdlib>
 
 namespace {
 
@@ -405,18 +406,247 @@ hipError_t gesl_sweeps_split(const float* A, int64_t lda, int64_t n, int B, cons
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Single-launch substitution (one right-hand side, whole diagonal-block
+// inverses).  The entries are cut into slices of S = min(64, B); slice s
+// belongs to workgroup s mod G, which keeps its running sum z_s in LDS for the
+// whole launch.  Forward sweep, block step q ascending: every slice beyond
+// block q takes block q's contribution z_s += A[qB .. qB+B)[slice]^T y_q as
+// soon as y_q is published; a slice of block q+1 then publishes z_s and, once
+// the T = B / S slices of its block have, computes its part of
+// y_{q+1} = W_L(q+1)^T z_{q+1} from the block's inverse and publishes it.  The
+// backward sweep mirrors it (descending, U and -W_U).  Each workgroup handles
+// its slices in block order, so the slice on the critical path goes first.
+// Publication: values are written with agent-scope coherent stores, every
+// wave drains its stores, then the slice's flag is set (1 forward, 2
+// backward; a small kernel clears the flags before each launch).  A waiter
+// polls the T flags of the block it needs, bounded: past the limit the call
+// fails with status bit 2 and every workgroup leaves.  The factor tile of a
+// contribution and the inverse tile of a diagonal solve do not depend on the
+// awaited values, so they are loaded before the wait and only the flag
+// latency and the arithmetic stay on the critical path.
+// ---------------------------------------------------------------------------
+constexpr int kMaxOwn = 16;            // slices per workgroup (LDS capacity)
+constexpr long kGeslSpinLimit = 1L << 22;
+
+__device__ __forceinline__ void pub_f(float* p, float v) {
+    __hip_atomic_store(reinterpret_cast<unsigned*>(p), __float_as_uint(v), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ float get_f(const float* p) {
+    return __uint_as_float(__hip_atomic_load(reinterpret_cast<const unsigned*>(p),
+                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+
+__global__ void gesl_persist_reset_kernel(unsigned* flags, int64_t nflags, int* status) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nflags;
+         i += (int64_t)gridDim.x * blockDim.x)
+        flags[i] = 0u;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *status = 0;
+}
+
+template <int B>
+__global__ __launch_bounds__(NT) void gesl_persist_kernel(const float* __restrict__ A, int64_t lda,
+                                                          int64_t n, const float* __restrict__ binv,
+                                                          float* __restrict__ x, GeslWs ws,
+                                                          long spin_limit) {
+    constexpr int S = B < 64 ? B : 64;  // slice width
+    constexpr int T = B / S;            // slices per block
+    constexpr int CG = S / 4;           // float4 column groups of a slice
+    constexpr int RG = NT / CG;         // row groups
+    constexpr int RPT = B / RG;         // rows of a tile per thread
+    static_assert(RPT >= 1 && RPT * RG == B, "tile split");
+    __shared__ float zown[kMaxOwn * S];  // running sums of the owned slices
+    __shared__ float vbuf[B];            // the awaited block vector
+    __shared__ float red[RG * S];
+    __shared__ int abort_flag;
+    const int t = threadIdx.x, G = gridDim.x, g = blockIdx.x;
+    const int NS = (int)(n / S), N = (int)(n / B);
+    const int own = (NS - g + G - 1) / G;  // slices g, g + G, ...
+    const int cg = t % CG, rg = t / CG;
+    unsigned* zflag = ws.flags;
+    unsigned* xflag = ws.flags + NS;
+
+    for (int k = 0; k < own; ++k)
+        for (int c = t; c < S; c += NT) zown[k * S + c] = x[(int64_t)(g + k * G) * S + c];
+
+    f32x4 tile[RPT];
+    // rows p = rg + RG i (stride ld) of base, the 4 columns of group cg
+    auto load_tile = [&](const float* base, int64_t ld) {
+#pragma unroll
+        for (int i = 0; i < RPT; ++i)
+            tile[i] = *reinterpret_cast<const f32x4*>(base + (int64_t)(rg + RG * i) * ld + 4 * cg);
+    };
+    // out[c] = (add ? out[c] : 0) + sign * sum_p tile[p][c] vbuf[p]
+    auto gemv = [&](float* out, float sign, bool add) {
+        f32x4 a = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int i = 0; i < RPT; ++i) {
+            const float v = vbuf[rg + RG * i];
+            a[0] = fmaf(tile[i][0], v, a[0]);
+            a[1] = fmaf(tile[i][1], v, a[1]);
+            a[2] = fmaf(tile[i][2], v, a[2]);
+            a[3] = fmaf(tile[i][3], v, a[3]);
+        }
+        *reinterpret_cast<f32x4*>(&red[rg * S + 4 * cg]) = a;
+        __syncthreads();
+        if (t < S) {
+            float sum = 0.0f;
+#pragma unroll 8
+            for (int r = 0; r < RG; ++r) sum += red[r * S + t];
+            out[t] = add ? out[t] + sign * sum : sign * sum;
+        }
+        __syncthreads();
+    };
+    // wait until the T flags from f on are >= target; true: the call failed
+    auto wait = [&](const unsigned* f, unsigned target) {
+        if (t == 0) {
+            int ab = 0;
+            long spins = 0;
+            for (int m = 0; m < T && !ab; ++m)
+                while (__hip_atomic_load(f + m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
+                       target) {
+                    if (++spins > spin_limit ||
+                        (__hip_atomic_load(ws.status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) &
+                         2)) {
+                        atomicOr(ws.status, 2);
+                        ab = 1;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            abort_flag = ab;
+        }
+        __syncthreads();
+        return abort_flag != 0;
+    };
+    // S values of LDS src to dst, then the slice's flag
+    auto publish = [&](float* dst, const float* src, unsigned* flag, unsigned val) {
+        if (t < S) pub_f(dst + t, src[t]);
+        __builtin_amdgcn_s_waitcnt(0);  // this wave's stores have completed
+        __syncthreads();
+        if (t == 0) __hip_atomic_store(flag, val, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    };
+    auto fetch_vec = [&](const float* src) {
+        for (int p = t; p < B; p += NT) vbuf[p] = get_f(src + p);
+        __syncthreads();
+    };
+    // block Q's diagonal solve for owned slice k (= s): forward y_s, backward x_s
+    auto diag = [&](int Q, int k, int s, bool fwd) {
+        float* z = zown + k * S;
+        const unsigned val = fwd ? 1u : 2u;
+        float* zb = fwd ? ws.zf : ws.zb;
+        publish(zb + (int64_t)s * S, z, zflag + s, val);
+        load_tile(binv + (int64_t)Q * 2 * B * B + (fwd ? 0 : (int64_t)B * B) + (s - Q * T) * S, B);
+        if (wait(zflag + (int64_t)Q * T, val)) return true;
+        fetch_vec(zb + (int64_t)Q * B);
+        gemv(z, fwd ? 1.0f : -1.0f, false);
+        publish((fwd ? ws.y : x) + (int64_t)s * S, z, xflag + s, val);
+        return false;
+    };
+    // block q's contribution to owned slice k (= s)
+    auto contrib = [&](int q, int k, int s, bool fwd) {
+        load_tile(A + (int64_t)q * B * lda + (int64_t)s * S, lda);
+        if (wait(xflag + (int64_t)q * T, fwd ? 1u : 2u)) return true;
+        fetch_vec((fwd ? ws.y : x) + (int64_t)q * B);
+        gemv(zown + k * S, fwd ? 1.0f : -1.0f, true);
+        return false;
+    };
+    __syncthreads();
+
+    // forward: L y = b
+    if (g < T && diag(0, 0, g, true)) return;
+    for (int q = 0; q + 1 < N; ++q)
+        for (int k = 0; k < own; ++k) {
+            const int s = g + k * G, blk = s / T;
+            if (blk <= q) continue;
+            if (contrib(q, k, s, true)) return;
+            if (blk == q + 1 && diag(q + 1, k, s, true)) return;
+        }
+    // backward: U x = y (z of each owned slice starts at its y)
+    for (int k = own - 1; k >= 0; --k) {
+        const int s = g + k * G;
+        if (s / T == N - 1 && diag(N - 1, k, s, false)) return;
+    }
+    for (int q = N - 1; q >= 1; --q)
+        for (int k = own - 1; k >= 0; --k) {
+            const int s = g + k * G, blk = s / T;
+            if (blk >= q) continue;
+            if (contrib(q, k, s, false)) return;
+            if (blk == q - 1 && diag(q - 1, k, s, false)) return;
+        }
+}
+
+// Workgroups of the single-launch substitution of order n at block B, or 0
+// when it does not apply (then the per-step launches solve).  At most half of
+// what the device holds at once, so that the launch stays co-resident beside
+// other work on the device.
+int64_t persist_groups(int64_t n, int B) {
+    if (B < 32 || B > 256 || (B & (B - 1)) || n % B) return 0;
+    const int S = std::min(64, B);
+    const int64_t NS = n / S;
+    static int cap[4] = {0, 0, 0, 0};
+    const int bi = B == 32 ? 0 : B == 64 ? 1 : B == 128 ? 2 : 3;
+    if (!cap[bi]) {
+        const void* k = B == 32    ? reinterpret_cast<const void*>(gesl_persist_kernel<32>)
+                        : B == 64  ? reinterpret_cast<const void*>(gesl_persist_kernel<64>)
+                        : B == 128 ? reinterpret_cast<const void*>(gesl_persist_kernel<128>)
+                                   : reinterpret_cast<const void*>(gesl_persist_kernel<256>);
+        int per_cu = 0, dev = 0, cus = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, NT, 0) != hipSuccess ||
+            hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+            (void)hipGetLastError();
+            return 0;
+        }
+        cap[bi] = std::max(1, per_cu * cus / 2);
+    }
+    int64_t G = std::min<int64_t>(NS, cap[bi]);
+    if (const char* e = getenv("HPL_GESL_GROUPS"))
+        G = std::max<int64_t>(1, std::min<int64_t>(G, atoll(e)));
+    // a block's slices on distinct workgroups; the owned slices fit the LDS
+    if (G < B / S || (NS + G - 1) / G > kMaxOwn) return 0;
+    return G;
+}
+
+hipError_t gesl_persist(const float* A, int64_t lda, int64_t n, int B, const float* binv,
+                        float* x, const GeslWs& ws, int64_t G, hipStream_t s) {
+    const int64_t NS = n / std::min(64, B);
+    gesl_persist_reset_kernel<<<(unsigned)std::min<int64_t>((2 * NS + 255) / 256, 64), 256, 0,
+                                s>>>(ws.flags, 2 * NS, ws.status);
+    const unsigned grid = (unsigned)G;
+    switch (B) {
+        case 32:
+            gesl_persist_kernel<32><<<grid, NT, 0, s>>>(A, lda, n, binv, x, ws, kGeslSpinLimit);
+            break;
+        case 64:
+            gesl_persist_kernel<64><<<grid, NT, 0, s>>>(A, lda, n, binv, x, ws, kGeslSpinLimit);
+            break;
+        case 128:
+            gesl_persist_kernel<128><<<grid, NT, 0, s>>>(A, lda, n, binv, x, ws, kGeslSpinLimit);
+            break;
+        default:
+            gesl_persist_kernel<256><<<grid, NT, 0, s>>>(A, lda, n, binv, x, ws, kGeslSpinLimit);
+            break;
+    }
+    return hipGetLastError();
+}
+
 }  // namespace
 
 hipError_t launch_gesl_multi(const float* A, int64_t lda, int64_t n, int B, const float* inv,
                              int64_t inv_stride, float* X, int64_t ldx, int nrhs, hipStream_t s,
-                             const float* binv) {
+                             const float* binv, const GeslWs* ws) {
     if (B > 512 || B < 32 || B % 32 || n % B || nrhs < 1 || (nrhs > 1 && (ldx < n || ldx % 4)))
         return hipErrorInvalidValue;
     for (int r0 = 0; r0 < nrhs; r0 += 16) {
         const int nr = std::min(16, nrhs - r0);
         float* Xr = X + (int64_t)r0 * ldx;
         hipError_t e;
-        if (nr == 1 && binv && B <= 256)
+        const int64_t G = nr == 1 && binv && ws && (lda & 3) == 0 ? persist_groups(n, B) : 0;
+        if (G > 0)
+            e = gesl_persist(A, lda, n, B, binv, Xr, *ws, G, s);
+        else if (nr == 1 && binv && B <= 256)
             e = gesl_sweeps_split(A, lda, n, B, binv, Xr, s);
         else if (nr == 1)
             e = gesl_sweeps<1>(A, lda, n, B, inv, inv_stride, binv, Xr, ldx, 1, s);
@@ -430,6 +660,25 @@ hipError_t launch_gesl_multi(const float* A, int64_t lda, int64_t n, int B, cons
 }
 
 hipError_t launch_gesl(const float* A, int64_t lda, int64_t n, int B, const float* inv,
-                       int64_t inv_stride, float* x, hipStream_t s, const float* binv) {
-    return launch_gesl_multi(A, lda, n, B, inv, inv_stride, x, n, 1, s, binv);
+                       int64_t inv_stride, float* x, hipStream_t s, const float* binv,
+                       const GeslWs* ws) {
+    return launch_gesl_multi(A, lda, n, B, inv, inv_stride, x, n, 1, s, binv, ws);
+}
+
+int64_t gesl_persist_groups(int64_t n, int B) { return persist_groups(n, B); }
+
+size_t gesl_ws_bytes(int64_t n) {
+    // zf, zb, y (n floats each), two flags per slice of >= 32 entries, status
+    return (size_t)(3 * n) * sizeof(float) + (size_t)(2 * (n / 32) + 64) * sizeof(unsigned);
+}
+
+GeslWs gesl_ws_carve(void* base, int64_t n) {
+    GeslWs w;
+    float* f = static_cast<float*>(base);
+    w.zf = f;
+    w.zb = f + n;
+    w.y = f + 2 * n;
+    w.flags = reinterpret_cast<unsigned*>(f + 3 * n);
+    w.status = reinterpret_cast<int*>(w.flags + 2 * (n / 32));
+    return w;
 }

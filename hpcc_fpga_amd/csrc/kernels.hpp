@@ -68,15 +68,34 @@ hipError_t launch_block_inverse(const float* A, int64_t lda, int64_t n, int B, c
                                 int64_t inv_stride, float* out, hipStream_t s);
 
 // substitution (gesl.hip)
+// Workspace of the single-launch substitution of one right-hand side: the
+// forward and backward slice sums zf, zb and the forward solution y (n floats
+// each), two flags per slice and a status word (bit 2: a bounded wait gave
+// up).  Every launch clears the flags and the status itself.
+struct GeslWs {
+    float* zf;
+    float* zb;
+    float* y;
+    unsigned* flags;
+    int* status;
+};
+size_t gesl_ws_bytes(int64_t n);
+GeslWs gesl_ws_carve(void* base, int64_t n);  // base: gesl_ws_bytes(n), 16-byte aligned
+// workgroups of the single launch for order n at block B; 0: it does not
+// apply (block above 256, or too many slices per workgroup)
+int64_t gesl_persist_groups(int64_t n, int B);
 // binv: the full diagonal-block inverses of launch_block_inverse (B <= 256),
-// or null for the sequential diagonal solve with the 32 x 32 inverses
+// or null for the sequential diagonal solve with the 32 x 32 inverses.  With
+// binv, one right-hand side and a workspace, the substitution is a single
+// persistent launch; otherwise two launches per block step.
 hipError_t launch_gesl(const float* A, int64_t lda, int64_t n, int B, const float* inv,
-                       int64_t inv_stride, float* x, hipStream_t s, const float* binv = nullptr);
+                       int64_t inv_stride, float* x, hipStream_t s, const float* binv = nullptr,
+                       const GeslWs* ws = nullptr);
 // nrhs right-hand sides, column r at X + r * ldx (ldx a multiple of 4, >= n);
 // blocks up to 256 for more than 4 right-hand sides
 hipError_t launch_gesl_multi(const float* A, int64_t lda, int64_t n, int B, const float* inv,
                              int64_t inv_stride, float* X, int64_t ldx, int nrhs, hipStream_t s,
-                             const float* binv = nullptr);
+                             const float* binv = nullptr, const GeslWs* ws = nullptr);
 
 // partial pivoting (pivot.hip); X is the transposed matrix X[i][j] = M(i, j)
 hipError_t launch_transpose_square(float* A, int64_t ld, int64_t n, hipStream_t s);

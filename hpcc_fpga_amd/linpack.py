@@ -161,6 +161,7 @@ def load_library():
         "hpl_set_profiling": ([P, i32], i32),
         "hpl_kernel_stats": ([P, C.POINTER(u64), C.POINTER(dbl), C.POINTER(dbl)], i32),
         "hpl_kernel_busy": ([P, C.POINTER(dbl)], i32),
+        "hpl_kernel_bytes": ([P, C.POINTER(dbl)], i32),
         "hpl_gesl_noalloc": ([P, u64, u64, P, P], i32),
     }
     for name, (args, res) in sigs.items():
@@ -442,6 +443,12 @@ class Context:
         n, s, f = C.c_uint64(), C.c_double(), C.c_double()
         self._check(self._L.hpl_kernel_stats(self._c, C.byref(n), C.byref(s), C.byref(f)))
         return int(n.value), float(s.value), float(f.value)
+
+    def kernel_bytes(self) -> float:
+        """Algorithmic HBM bytes of the profiled trailing-update launches."""
+        b = C.c_double()
+        self._check(self._L.hpl_kernel_bytes(self._c, C.byref(b)))
+        return float(b.value)
 
     def kernel_busy(self) -> float:
         """Seconds during which at least one profiled trailing-update launch ran."""

@@ -359,6 +359,10 @@ hpl_status hpl_kernel_stats(hpl_ctx* ctx, uint64_t* launches, double* seconds, d
  * their intervals.  Launches on the two streams overlap, so this is below
  * the summed duration; flops / busy is the device's trailing-update rate. */
 hpl_status hpl_kernel_busy(hpl_ctx* ctx, double* seconds);
+/* Algorithmic HBM bytes of those launches: per launch the updated region of C
+ * read and written once (8 bytes per element) and each operand strip read
+ * once (K values per row, in the update's input precision). */
+hpl_status hpl_kernel_bytes(hpl_ctx* ctx, double* bytes);
 
 /* ---- substitution that allocates nothing ----------------------------------
  * No counterpart in the reference.  For callers that cannot allocate device
@@ -371,10 +375,10 @@ hpl_status hpl_kernel_busy(hpl_ctx* ctx, double* seconds);
  * the call only enqueues work, so synchronise that stream before reading x.
  * The block is hpl_gesl's (the largest power of two <= 256 dividing n; n must
  * be a multiple of 32) and the result equals hpl_gesl's bit for bit.  The
- * diagonal-block inverses, n * (384 + 8 * block) bytes, live in one fixed
- * 40 MiB device array per GPU (held until the process ends): a larger system
- * returns HPL_ERR_BUDGET before anything is enqueued (n <= 17152 when 256
- * divides n).  Every call on a GPU shares that array and overwrites it, so
+ * diagonal-block inverses and the solve's state, about n * (396 + 8 * block)
+ * bytes, live in one fixed 40 MiB device array per GPU (held until the
+ * process ends): a larger system returns HPL_ERR_BUDGET before anything is
+ * enqueued (n <= 17152 when 256 divides n).  Every call on a GPU shares that array and overwrites it, so
  * calls that overlap on one GPU (several streams or host threads) race. */
 hpl_status hpl_gesl_noalloc(const float* lu, uint64_t n, uint64_t lda, float* x, void* stream);
 

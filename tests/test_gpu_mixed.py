@@ -38,7 +38,7 @@ def test_lowprec_factor_close_to_fp32(ctx, prec, tol):
 
 
 @pytest.mark.parametrize("prec,n,B", [("bf16", 512, 64), ("fp16", 512, 64), ("fp8", 512, 64),
-                                      ("fp8", 768, 128)])
+                                      ("fp8", 768, 128), ("bf16", 1024, 256), ("fp16", 1536, 256)])
 def test_lowprec_factor_matches_quantised_emulation(ctx, prec, n, B):
     print(f"quantised emulation {prec} n={n} B={B}", flush=True)
     A, _, _, _ = O.generate_ref(n, seed=9)  # positive entries: coherent, large updates
