@@ -248,15 +248,18 @@ __device__ __forceinline__ int swz(int r) {
     else return ((r >> 1) ^ (r >> 2)) & 3;
 }
 
-// s_waitcnt immediate that waits until at most n vector memory operations of
-// this wave are outstanding (expcnt and lgkmcnt not waited for)
-constexpr int vmcnt_le(int n) { return (n & 15) | ((n >> 4) << 14) | (7 << 4) | (15 << 8); }
+// s_waitcnt immediate that waits until at most n vector memory operations and
+// at most lgkm LDS / scalar memory operations of this wave are outstanding
+// (expcnt not waited for)
+constexpr int vmcnt_le(int n, int lgkm = 15) {
+    return (n & 15) | ((n >> 4) << 14) | (7 << 4) | (lgkm << 8);
+}
 
 template <typename T, int TBM, int TBN, int BK, int NBUF>
 __global__ __launch_bounds__((TBM / 128) * (TBN / 128) * 64, 1) void update_lpw_kernel(
     float* __restrict__ C, int64_t ldc, const T* __restrict__ L, int64_t l_ld, int64_t l_slot,
     const T* __restrict__ Tt, int64_t t_ld, int64_t t_slot, int M, int N, int K, int Bblk,
-    LpRegion r2) {
+    LpRegion r2, int prefetch) {
     typedef typename lp_traits<T>::vec vec;
     constexpr int WM = 128, WN = 128, WAVES_M = TBM / WM, WAVES = WAVES_M * (TBN / WN);
     constexpr int FM = WM / 32, FN = WN / 32;
@@ -317,34 +320,64 @@ __global__ __launch_bounds__((TBM / 128) * (TBN / 128) * 64, 1) void update_lpw_
         }
     };
 
+    // L2 prefetch of the slab NBUF slabs ahead: one 4-byte DMA per 128-byte
+    // row segment of that slab (waves 0-1 the L rows, waves 2-3 the T rows,
+    // two rows per lane) into a scratch area of LDS nobody reads, so that the
+    // slab's own DMA, issued a slab later, finds its rows in L2.  A DMA
+    // returns no register, so the prefetch never holds up the MFMAs; it only
+    // counts in vmcnt, which the waits below allow for.
+    static_assert(TBM == 256 && TBN == 256, "the prefetch mapping assumes 256 x 256 tiles");
+    __shared__ __attribute__((aligned(16))) unsigned pf_sink[4][2][64];
+    const bool pf_on = prefetch != 0;
+    auto prefetch_slab = [&](int k0) {
+        const int64_t ks = k0 / Bblk, ko = k0 % Bblk;
+        const int r = (wave & 1) * 128 + lane;
+        const T* row = wave < 2 ? L + ks * l_slot + ko + (int64_t)(m0 + r) * l_ld
+                                : Tt + ks * t_slot + ko + (int64_t)(n0 + r) * t_ld;
+        const int64_t ld64 = (wave < 2 ? l_ld : t_ld) * 64;  // 64 rows further
+        __builtin_amdgcn_global_load_lds(reinterpret_cast<const unsigned*>(row),
+                                         (__attribute__((address_space(3))) void*)(&pf_sink[wave][0][0]),
+                                         4, 0, 0);
+        __builtin_amdgcn_global_load_lds(reinterpret_cast<const unsigned*>(row + ld64),
+                                         (__attribute__((address_space(3))) void*)(&pf_sink[wave][1][0]),
+                                         4, 0, 0);
+    };
+    constexpr int PFN = 2;  // prefetch DMAs per wave and slab
+
     const int nk = K / BK;
     // the first NBUF - 1 slabs go out before the loads of C
 #pragma unroll
     for (int p = 0; p + 1 < NBUF; ++p)
         if (p < nk) dma_slab(p * BK, p);
 
-    // the accumulators start from C (the same MFMA sequence as the 128-tile
-    // kernel, so both give the same bits): register r of block (i, j) holds
-    // row rbase + 32 i + 4 (lane / 32) + 8 (r / 4) + r % 4, column cbase +
-    // 32 j + lane % 32 -- one row pointer per (i, r), the four blocks at
-    // immediate offsets.  The row stride, read back from LDS, lives in a VGPR,
-    // so the row offsets are formed per use rather than kept as scalars.
+    // The MFMAs take T as their A operand and L as their B operand, so each
+    // 32 x 32 accumulator block holds a block of C transposed: register
+    // 4 q + e of block (i, j) is C row rbase + 32 i + lane % 32, column
+    // cbase + 32 j + 8 q + 4 (lane / 32) + e.  Four consecutive registers are
+    // four consecutive floats of a row, so C moves as 16-byte vectors: one
+    // row pointer per i, the 16 vectors of a row at immediate offsets.  (The
+    // products and their k order are those of the 128-tile kernel, which puts
+    // L first; the accumulators start from C in both, so both give the same
+    // bits.)  The row stride, read back from LDS, lives in a VGPR, so the row
+    // offsets are formed per use rather than kept as scalars.
     const int rbase = m0 + wm * WM, cbase = n0 + wn * WN;
     __shared__ int sh_ldc;
     if (tid == 0) sh_ldc = (int)ldc;
     __syncthreads();
     const int ldc_v = sh_ldc;
-    float* cb = C + (int64_t)(rbase + 4 * lh) * ldc + cbase + lr;
+    float* cb = C + (int64_t)(rbase + lr) * ldc + cbase + 4 * lh;
     f32x16 acc[FM][FN];
 #pragma unroll
     for (int i = 0; i < FM; ++i) {
+        const float* p = cb + (int64_t)(i * 32) * ldc_v;
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            const float* p = cb + (int64_t)(i * 32 + 8 * (r >> 2) + (r & 3)) * ldc_v;
+        for (int j = 0; j < FN; ++j)
 #pragma unroll
-            for (int j = 0; j < FN; ++j) acc[i][j][r] = p[j * 32];
-        }
-        __builtin_amdgcn_sched_barrier(0);
+            for (int q = 0; q < 4; ++q) {
+                const f32x4 v = *reinterpret_cast<const f32x4*>(p + j * 32 + 8 * q);
+#pragma unroll
+                for (int e = 0; e < 4; ++e) acc[i][j][4 * q + e] = v[e];
+            }
     }
 
     u32x4 a[2][FM], b[2][FN];
@@ -361,8 +394,8 @@ __global__ __launch_bounds__((TBM / 128) * (TBN / 128) * 64, 1) void update_lpw_
         for (int i = 0; i < FM; ++i)
 #pragma unroll
             for (int j = 0; j < FN; ++j)
-                acc[i][j] = lp_traits<T>::mfma(__builtin_bit_cast(vec, a[slot][i]),
-                                               __builtin_bit_cast(vec, b[slot][j]), acc[i][j]);
+                acc[i][j] = lp_traits<T>::mfma(__builtin_bit_cast(vec, b[slot][j]),
+                                               __builtin_bit_cast(vec, a[slot][i]), acc[i][j]);
     };
 
     __builtin_amdgcn_s_waitcnt(0);  // C and the first slab
@@ -373,37 +406,61 @@ __global__ __launch_bounds__((TBM / 128) * (TBN / 128) * 64, 1) void update_lpw_
         // the buffer of slab kc - 1: every wave finished reading it before the
         // barrier that ended that slab
         if (kc + NBUF - 1 < nk) dma_slab((kc + NBUF - 1) * BK, (kc + NBUF - 1) % NBUF);
+        const bool pf_now = pf_on && kc + NBUF < nk;
+        if (pf_now) prefetch_slab((kc + NBUF) * BK);
 #pragma unroll
         for (int ks = 0; ks + 1 < KS; ++ks) {
             read_ops(buf, ks + 1, (ks & 1) ^ 1);
             mfmas(ks & 1);
         }
+        // Each k-step's LDS reads go out between its predecessor's MFMAs,
+        // after the first of them: the wait for a step's operands then sits
+        // in front of that step's first MFMA, a whole step after the reads
+        // were issued.  (While a DMA is in flight the compiler can only wait
+        // for all LDS reads at once, so reads issued right before that wait
+        // would stall the MFMAs for their full latency.)
 #pragma unroll
         for (int ks = 0; ks + 1 < KS; ++ks) {
-            __builtin_amdgcn_sched_group_barrier(0x100, FM + FN, 0);  // DS read
-            __builtin_amdgcn_sched_group_barrier(0x008, FM * FN, 0);  // MFMA
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+#pragma unroll
+            for (int q = 0; q < FM + FN; ++q) {
+                __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // DS read
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+            }
+            __builtin_amdgcn_sched_group_barrier(0x008, FM * FN - 1 - (FM + FN), 0);
         }
         if (kc + 1 < nk) {
-            // slab kc + 1 landed: at most the DMA of the slabs after it in flight
-            if (kc + NBUF - 1 < nk)
-                __builtin_amdgcn_s_waitcnt(vmcnt_le((NBUF - 2) * (IA + IB)));
-            else
+            // slab kc + 1 landed -- at most the DMA of the slabs after it and
+            // this slab's prefetch (issued after them) still in flight -- and
+            // this wave's LDS reads of slab kc are done; then a bare barrier
+            // (no fence: that would also wait for the prefetch)
+            if (kc + NBUF - 1 < nk) {
+                if (pf_now)
+                    __builtin_amdgcn_s_waitcnt(vmcnt_le((NBUF - 2) * (IA + IB) + PFN, 0));
+                else
+                    __builtin_amdgcn_s_waitcnt(vmcnt_le((NBUF - 2) * (IA + IB), 0));
+            } else {
                 __builtin_amdgcn_s_waitcnt(0);
-            __syncthreads();
+            }
+            __builtin_amdgcn_s_barrier();
             read_ops((kc + 1) % NBUF, 0, KS & 1);
         }
         mfmas((KS - 1) & 1);
     }
+    __builtin_amdgcn_s_waitcnt(0);
 
 #pragma unroll
     for (int i = 0; i < FM; ++i) {
+        float* p = cb + (int64_t)(i * 32) * ldc_v;
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            float* p = cb + (int64_t)(i * 32 + 8 * (r >> 2) + (r & 3)) * ldc_v;
+        for (int j = 0; j < FN; ++j)
 #pragma unroll
-            for (int j = 0; j < FN; ++j) p[j * 32] = acc[i][j][r];
-        }
-        __builtin_amdgcn_sched_barrier(0);
+            for (int q = 0; q < 4; ++q) {
+                f32x4 v;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) v[e] = acc[i][j][4 * q + e];
+                *reinterpret_cast<f32x4*>(p + j * 32 + 8 * q) = v;
+            }
     }
 }
 
@@ -411,15 +468,26 @@ __global__ __launch_bounds__((TBM / 128) * (TBN / 128) * 64, 1) void update_lpw_
 // 128 x 128 one (for measurements).
 // Slab depth of the wide kernel: 64 k values in two buffers (default) or 32
 // in four (HPL_LP_SLAB=32, three slabs of DMA ahead instead of one).  At
-// M = N = 16384 the first runs at 927 / 984 / 716 TFLOP/s for K = 3072 / 6144 /
-// 1536, the second at 713 / 761 / 570: the extra barriers of the shorter slabs
-// cost more than the deeper prefetch gains.
+// M = N = 16384 the first runs at 950 / 1109 / 729 TFLOP/s for K = 3072 /
+// 6144 / 1536 (928 / 993 / 710 without the L2 prefetch); the second ran at
+// 713 / 761 / 570: the extra barriers of the shorter slabs cost more than the
+// deeper prefetch gains.
 int lp_wide_slab() {
     static const int k = [] {
         const char* e = getenv("HPL_LP_SLAB");
         return e && atoi(e) == 32 ? 32 : 64;
     }();
     return k;
+}
+
+// L2 prefetch of the wide kernel's slabs (HPL_LP_PREFETCH=0 turns it off,
+// for measurements).
+int lp_prefetch() {
+    static const int p = [] {
+        const char* e = getenv("HPL_LP_PREFETCH");
+        return e && atoi(e) == 0 ? 0 : 1;
+    }();
+    return p;
 }
 
 bool lp_wide() {
@@ -439,17 +507,20 @@ hipError_t launch_t(float* C, int64_t ldc, const void* L, int64_t l_ld, int64_t 
     if (!order_regions(C, L, Tt, scL, scT, M, N, r2)) return hipSuccess;
     if (K % BK || Bblk % BK || (l_ld % 8) || (t_ld % 8)) return hipErrorInvalidValue;
     constexpr int W = 256;
-    if (lp_wide() && M % W == 0 && N % W == 0 && r2.M % W == 0 && r2.N % W == 0) {
+    // the wide kernel moves C as 16-byte row vectors
+    const bool c_vec = (ldc % 4) == 0 && (reinterpret_cast<uintptr_t>(C) & 15) == 0 &&
+                       (reinterpret_cast<uintptr_t>(r2.C) & 15) == 0;
+    if (lp_wide() && c_vec && M % W == 0 && N % W == 0 && r2.M % W == 0 && r2.N % W == 0) {
         const int tiles = ((M + W - 1) / W) * ((N + W - 1) / W) +
                           (r2.M > 0 ? ((r2.M + W - 1) / W) * ((r2.N + W - 1) / W) : 0);
         if (lp_wide_slab() == 64)
             update_lpw_kernel<T, W, W, 64, 2><<<tiles, (W / 128) * (W / 128) * 64, 0, s>>>(
                 C, ldc, static_cast<const T*>(L), l_ld, l_slot, static_cast<const T*>(Tt), t_ld,
-                t_slot, M, N, K, Bblk, r2);
+                t_slot, M, N, K, Bblk, r2, lp_prefetch());
         else
             update_lpw_kernel<T, W, W, 32, 4><<<tiles, (W / 128) * (W / 128) * 64, 0, s>>>(
                 C, ldc, static_cast<const T*>(L), l_ld, l_slot, static_cast<const T*>(Tt), t_ld,
-                t_slot, M, N, K, Bblk, r2);
+                t_slot, M, N, K, Bblk, r2, lp_prefetch());
         return hipGetLastError();
     }
     const int tiles = region_tiles(M, N) + region_tiles(r2.M, r2.N);
