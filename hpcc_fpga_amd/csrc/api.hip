@@ -127,6 +127,8 @@ hpl_status alloc_factor_ws(hpl_ctx* c, uint32_t B) {
     if (st) return st;
     st = ensure(c, c->dInner, c->capInner, w.inner);
     if (st) return st;
+    st = ensure(c, c->dTinv, c->capTinv, c->strip_inverse ? w.tinv : 0);
+    if (st) return st;
     st = ensure(c, c->dLpT, c->capLpT, w.lp);
     if (st) return st;
     st = ensure(c, c->dLpL, c->capLpL, w.lp);
@@ -159,15 +161,16 @@ struct UpdRegion {
 
 // Trailing update of one region, or of two in a single launch (b.M > 0).
 hpl_status launch_update2(hpl_ctx* c, hipStream_t st, int64_t k0, UpdRegion a, UpdRegion b, int K,
-                          uint32_t B);
+                          uint32_t B, int max_groups = 0);
 
 hpl_status launch_update(hpl_ctx* c, hipStream_t st, int64_t k0, int64_t r0, int64_t c0, int M,
-                         int N, int K, uint32_t B) {
-    return launch_update2(c, st, k0, UpdRegion{r0, c0, M, N}, UpdRegion{0, 0, 0, 0}, K, B);
+                         int N, int K, uint32_t B, int max_groups = 0) {
+    return launch_update2(c, st, k0, UpdRegion{r0, c0, M, N}, UpdRegion{0, 0, 0, 0}, K, B,
+                          max_groups);
 }
 
 hpl_status launch_update2(hpl_ctx* c, hipStream_t st, int64_t k0, UpdRegion a, UpdRegion b, int K,
-                          uint32_t B) {
+                          uint32_t B, int max_groups) {
     if (a.M <= 0 || a.N <= 0) {
         a = b;
         b = UpdRegion{0, 0, 0, 0};
@@ -214,7 +217,7 @@ hpl_status launch_update2(hpl_ctx* c, hipStream_t st, int64_t k0, UpdRegion a, U
         } else {
             HIPCHK(c, launch_gemm_update_lp(c->prec, C, lda, Lb + r0 * B * esz, B, n * B,
                                             Tb + c0 * B * esz, B, n * B, M, N, K, (int)B, st,
-                                            &r2));
+                                            &r2, max_groups));
         }
     }
     if (c->prof) {
@@ -249,6 +252,16 @@ hpl_status factor_diag(hpl_ctx* c, hipStream_t st, int64_t off, uint32_t B, floa
     return factor_diag(c, st, off + h, h, next);
 }
 
+// Step k's buffer of diagonal-block inverses for the strip solves of the
+// BF16 factorisation, or null where the strips take the recurrence.  (The
+// split product is exact to about 2^-17, far below BF16's rounding of the
+// update operands (2^-9) but not far enough below FP16's (2^-12): the FP16
+// factorisation keeps the FP32 recurrence.)
+float* step_tinv(hpl_ctx* c, int64_t k, uint32_t B) {
+    if (!c->dTinv || !c->strip_inverse || B > 256 || c->prec != HPL_PREC_BF16) return nullptr;
+    return c->dTinv + (k & 1) * tinv_floats((int)B);
+}
+
 // Panel of step k, part 1: factor the diagonal block and prepare its
 // inverses (latency-bound, a few workgroups).
 hpl_status panel_diag(hpl_ctx* c, hipStream_t st, int64_t k, uint32_t B) {
@@ -259,6 +272,7 @@ hpl_status panel_diag(hpl_ctx* c, hipStream_t st, int64_t k, uint32_t B) {
     float* inv = c->dInv + k * (int64_t)(3 * (B / 32) * 1024);
     float* DT = c->dDT + (k & 1) * (int64_t)B * B;
     HIPCHK(c, launch_prep(c->dA, lda, off, B, DT, inv, st));
+    if (float* t = step_tinv(c, k, B)) HIPCHK(c, launch_tinv_prepare(c->dA, lda, off, B, inv, t, st));
     return HPL_SUCCESS;
 }
 
@@ -288,8 +302,11 @@ hpl_status panel_strips(hpl_ctx* c, hipStream_t st, int64_t k, uint32_t B) {
             scT = c->dScT + slot * (n / 32) + (off + B) / 32;
             scL = c->dScL + slot * (n / 32) + (off + B) / 32;
         }
-        HIPCHK(c, launch_trsm(c->dA, lda, off, m, B, DT, inv, Lt, n, st, c->prec, lpT, lpL, B, scT,
-                              scL));
+        if (const float* t = step_tinv(c, k, B))
+            HIPCHK(c, launch_trsm_inv(c->dA, lda, off, m, B, t, st, c->prec, lpT, lpL, B));
+        else
+            HIPCHK(c, launch_trsm(c->dA, lda, off, m, B, DT, inv, Lt, n, st, c->prec, lpT, lpL,
+                                  B, scT, scL));
     }
     return HPL_SUCCESS;
 }
@@ -388,13 +405,21 @@ hpl_status gefa_pipelined(hpl_ctx* c, uint32_t B) {
     hpl_status r;
     for (int64_t g = 0; g < G; ++g) {
         const int64_t s = bnd[g + 1] * (int64_t)B;
-        if (g >= 1 && s < n) {
+        const bool wide = (c->prec == HPL_PREC_BF16 || c->prec == HPL_PREC_FP16) &&
+                          (n - s) % 256 == 0;
+        if (g >= 1 && s < n && wide && c->chain_cus > 0 && c->num_cu > c->chain_cus) {
+            // one launch on at most num_cu - chain_cus persistent workgroups:
+            // the chain's launches always find chain_cus CUs free
+            const int64_t steps = bnd[g] - bnd[g - 1];
+            r = launch_update(c, c->s1, bnd[g - 1], s, s, (int)(n - s), (int)(n - s),
+                              (int)(steps * B), B, c->num_cu - c->chain_cus);
+            if (r) return r;
+        } else if (g >= 1 && s < n) {
             // group g-1's steps in column slices of whole tiles (256 wide for
             // the 16-bit update's 256 x 256 tiles, else 128)
             const int64_t steps = bnd[g] - bnd[g - 1];
             const int64_t slices = c->bulk_slices > 0 ? c->bulk_slices : steps;
-            const int64_t tw = (c->prec == HPL_PREC_BF16 || c->prec == HPL_PREC_FP16) && (n - s) % 256 == 0
-                                   ? 256 : 128;
+            const int64_t tw = wide ? 256 : 128;
             const int64_t w = n - s, tiles = w / tw;
             const int64_t S = std::max<int64_t>(1, std::min(slices, tiles));
             for (int64_t i = 0; i < S; ++i) {
@@ -409,13 +434,28 @@ hpl_status gefa_pipelined(hpl_ctx* c, uint32_t B) {
         const int64_t k0 = g >= 1 ? bnd[g - 1] : 0;
         for (int64_t k = bnd[g]; k < bnd[g + 1]; ++k) {
             const int64_t kb = k * (int64_t)B, m = n - kb;
-            if (k > k0) {
-                r = launch_update2(c, c->s0, k0, UpdRegion{kb, kb, (int)B, (int)m},
-                                   UpdRegion{kb + B, kb, (int)(m - B), (int)B},
-                                   (int)((k - k0) * B), B);
+            if (k == k0) {
+                r = panel(c, c->s0, k, B);
+                if (r) return r;
+                continue;
+            }
+            // The diagonal block's update and factorisation go first, on s2,
+            // while s0 updates the rest of the row and column blocks; the
+            // strip solves then wait for both.
+            const int K = (int)((k - k0) * B);
+            HIPCHK(c, hipEventRecord(c->evS, c->s0));
+            HIPCHK(c, hipStreamWaitEvent(c->s2, c->evS, 0));
+            r = launch_update(c, c->s2, k0, kb, kb, (int)B, (int)B, K, B);
+            if (!r) r = panel_diag(c, c->s2, k, B);
+            if (r) return r;
+            HIPCHK(c, hipEventRecord(c->evD, c->s2));
+            if (m > (int64_t)B) {
+                r = launch_update2(c, c->s0, k0, UpdRegion{kb, kb + B, (int)B, (int)(m - B)},
+                                   UpdRegion{kb + B, kb, (int)(m - B), (int)B}, K, B);
                 if (r) return r;
             }
-            r = panel(c, c->s0, k, B);
+            HIPCHK(c, hipStreamWaitEvent(c->s0, c->evD, 0));
+            r = panel_strips(c, c->s0, k, B);
             if (r) return r;
         }
         HIPCHK(c, hipEventRecord(c->evA, c->s0));
@@ -491,6 +531,9 @@ WsPlan plan_factor_ws(uint64_t n, uint32_t B, int prec, int depth, bool linear) 
     w.sc = prec == HPL_PREC_FP8 ? slots * (n / 32) * sizeof(float) : 0;
     w.status = 64;
     w.binv = B <= 256 ? 2 * (size_t)n * B * sizeof(float) : 0;
+    w.tinv = prec == HPL_PREC_BF16 && B <= 256
+                 ? 2 * (size_t)tinv_floats((int)B) * sizeof(float)
+                 : 0;
     return w;
 }
 
@@ -554,9 +597,12 @@ IrPlan plan_ir(uint64_t n, int nrhs) {
 hpl_status init_streams(hpl_ctx* c) {
     if (hipStreamCreateWithFlags(&c->s0, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&c->s1, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->s2, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->e0) != hipSuccess || hipEventCreate(&c->e1) != hipSuccess ||
         hipEventCreateWithFlags(&c->evA, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c->evP, hipEventDisableTiming) != hipSuccess)
+        hipEventCreateWithFlags(&c->evP, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->evS, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->evD, hipEventDisableTiming) != hipSuccess)
         return fail(c, HPL_ERR_HIP, "stream or event creation failed");
     return HPL_SUCCESS;
 }
@@ -673,6 +719,11 @@ hpl_status hpl_create(int device, hpl_ctx** out) {
     if (const char* e = getenv("HPL_LOOKAHEAD")) c->lookahead = atoi(e) != 0;
     if (const char* e = getenv("HPL_SCHEDULE")) c->pipelined = strcmp(e, "grouped") != 0;
     if (const char* e = getenv("HPL_BULK_SLICES")) c->bulk_slices = std::max(0, atoi(e));
+    if (const char* e = getenv("HPL_STRIP_SOLVE")) c->strip_inverse = strcmp(e, "recurrence") != 0;
+    if (const char* e = getenv("HPL_CHAIN_CUS")) c->chain_cus = std::max(0, atoi(e));
+    if (hipDeviceGetAttribute(&c->num_cu, hipDeviceAttributeMultiprocessorCount, device) !=
+        hipSuccess)
+        c->num_cu = 0;
     if (const char* e = getenv("HPL_TAIL_M")) c->tail_m = std::max(0LL, atoll(e));
     if (const char* e = getenv("HPL_TAIL_DEPTH")) c->tail_depth = std::max(0, std::min(16, atoi(e)));
     if (const char* e = getenv("HPL_PANEL_SPIN_LIMIT")) c->pv_spin_limit = atol(e);
@@ -692,10 +743,11 @@ void hpl_destroy(hpl_ctx* c) {
     (void)hipSetDevice(c->device);
     if (c->s0) (void)hipStreamSynchronize(c->s0);
     if (c->s1) (void)hipStreamSynchronize(c->s1);
+    if (c->s2) (void)hipStreamSynchronize(c->s2);
     if (!c->external)
         for (void* p : {(void*)c->dA, (void*)c->dA0, (void*)c->db, (void*)c->dx, (void*)c->db64,
                         (void*)c->dxt, (void*)c->dLt, (void*)c->dDT, (void*)c->dInv,
-                        (void*)c->dInner, (void*)c->dStatus, (void*)c->dWork, (void*)c->dOut4,
+                        (void*)c->dInner, (void*)c->dTinv, (void*)c->dStatus, (void*)c->dWork, (void*)c->dOut4,
                         c->dLpT, c->dLpL, (void*)c->dScT, (void*)c->dScL, (void*)c->dX64,
                         (void*)c->dIrR, (void*)c->dIrXs, (void*)c->dIrOut, (void*)c->dRhsB,
                         (void*)c->dRhsX, (void*)c->dGmV, (void*)c->dGmW, (void*)c->dGmC,
@@ -711,6 +763,9 @@ void hpl_destroy(hpl_ctx* c) {
     if (c->e1) (void)hipEventDestroy(c->e1);
     if (c->evA) (void)hipEventDestroy(c->evA);
     if (c->evP) (void)hipEventDestroy(c->evP);
+    if (c->evS) (void)hipEventDestroy(c->evS);
+    if (c->evD) (void)hipEventDestroy(c->evD);
+    if (c->s2) (void)hipStreamDestroy(c->s2);
     if (c->s1) (void)hipStreamDestroy(c->s1);
     if (c->s0) (void)hipStreamDestroy(c->s0);
     delete c;

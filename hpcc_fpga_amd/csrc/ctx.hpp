@@ -22,8 +22,10 @@ struct hpl_ctx {
     bool external = false;
     hipStream_t s0 = nullptr;   // trailing updates (and everything else)
     hipStream_t s1 = nullptr;   // look-ahead panel factorisation
+    hipStream_t s2 = nullptr;   // diagonal blocks of the pipelined schedule
     hipEvent_t e0 = nullptr, e1 = nullptr;
     hipEvent_t evA = nullptr, evP = nullptr;  // next-panel region updated / panel ready
+    hipEvent_t evS = nullptr, evD = nullptr;  // strips solved / diagonal block factorised
     bool lookahead = true;
     // Pipelined schedule (with look-ahead): the bulk update of one group of
     // `depth` steps runs on s1 while the next group's panels run on s0; the
@@ -33,6 +35,10 @@ struct hpl_ctx {
     // Column slices (separate launches) of each bulk update of the pipelined
     // schedule; 0: as many as the update depth.
     int bulk_slices = 0;
+    // Pipelined 16-bit schedule: CUs kept free of the bulk update for the
+    // chain (0: bulk in column slices instead); num_cu = the device's CUs.
+    int chain_cus = 0;
+    int num_cu = 0;
     // Tail of the pipelined schedule: once the order still to factorise is at
     // most tail_m (-1: half the order; 0: no tail), groups hold tail_depth
     // steps (0: half the depth, at least 1).  The row and column-block updates
@@ -81,6 +87,12 @@ struct hpl_ctx {
     bool binv_ready = false;    // dBInv holds the inverses of the current factor
     float* dInner = nullptr;    // recursion scratch for blocks > 128
     size_t capInner = 0;
+    // BF16 factorisation: the current steps' diagonal-block inverses for
+    // the strip solves (two buffers, step parity), and whether to use them
+    // (HPL_STRIP_SOLVE=recurrence: the 32 x 32 block recurrence instead)
+    float* dTinv = nullptr;
+    size_t capTinv = 0;
+    bool strip_inverse = true;
     void* dLpT = nullptr;       // low-precision k-contiguous top-strip copies, slots of n x B (as dLt)
     void* dLpL = nullptr;       // low-precision k-contiguous left-strip copies, same layout
     size_t capLpT = 0, capLpL = 0;
@@ -175,6 +187,7 @@ struct WsPlan {
     size_t lt, dt, inner, lp, sc, status;
     size_t inv;   // per-block 32 x 32 inverses (kept with the factor)
     size_t binv;  // full diagonal-block inverses for the substitution (kept with the factor)
+    size_t tinv;  // 16-bit factorisations: two steps' inverses for the strip solves
 };
 // `linear`: one strip slot per step (the pipelined schedule) instead of
 // 2 * depth reused slots.

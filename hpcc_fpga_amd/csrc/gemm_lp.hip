@@ -18,6 +18,7 @@
 #include "common.hpp"
 #include "kernels.hpp"
 
+#include <algorithm>
 #include <cstdlib>
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -259,7 +260,7 @@ template <typename T, int TBM, int TBN, int BK, int NBUF>
 __global__ __launch_bounds__((TBM / 128) * (TBN / 128) * 64, 1) void update_lpw_kernel(
     float* __restrict__ C, int64_t ldc, const T* __restrict__ L, int64_t l_ld, int64_t l_slot,
     const T* __restrict__ Tt, int64_t t_ld, int64_t t_slot, int M, int N, int K, int Bblk,
-    LpRegion r2, int prefetch) {
+    LpRegion r2) {
     typedef typename lp_traits<T>::vec vec;
     constexpr int WM = 128, WN = 128, WAVES_M = TBM / WM, WAVES = WAVES_M * (TBN / WN);
     constexpr int FM = WM / 32, FN = WN / 32;
@@ -272,11 +273,27 @@ __global__ __launch_bounds__((TBM / 128) * (TBN / 128) * 64, 1) void update_lpw_
     __shared__ __attribute__((aligned(16))) u32x4 As[NBUF][TBM * CPR];
     __shared__ __attribute__((aligned(16))) u32x4 Bs[NBUF][TBN * CPR];
 
+    // A grid smaller than the tile count makes the workgroups persistent:
+    // workgroup b takes tiles b, b + G, b + 2G, ... (G = gridDim.x, a multiple
+    // of 8, so every tile of a workgroup maps to the same XCD's share).
+    const int tiles1 = (M / TBM) * (N / TBN);
+    const int total = tiles1 + (r2.M > 0 ? (r2.M / TBM) * (r2.N / TBN) : 0);
+    float* const C1 = C;
+    const T* const L1 = L;
+    const T* const T1 = Tt;
+    const int M1 = M, N1 = N;
+    for (int tile = blockIdx.x; tile < total; tile += gridDim.x) {
+    if (tile != (int)blockIdx.x) __syncthreads();  // every wave done with the LDS slabs
+    C = C1;
+    L = L1;
+    Tt = T1;
+    M = M1;
+    N = N1;
     int tiles_m = M / TBM, tiles_n = N / TBN;
-    int bid = blockIdx.x, nblk = tiles_m * tiles_n;
+    int bid = tile, nblk = tiles1;
     if (bid >= nblk) {  // second region
         bid -= nblk;
-        nblk = gridDim.x - nblk;
+        nblk = total - tiles1;
         C = r2.C;
         L = static_cast<const T*>(r2.L);
         Tt = static_cast<const T*>(r2.T);
@@ -288,36 +305,38 @@ __global__ __launch_bounds__((TBM / 128) * (TBN / 128) * 64, 1) void update_lpw_
     int tm, tn;
     tile_coords(bid, nblk, tiles_m, tiles_n, tm, tn);
     const int m0 = tm * TBM, n0 = tn * TBN;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform: scalar LDS addresses
     const int wm = wave % WAVES_M, wn = wave / WAVES_M;
     const int lr = lane & 31, lh = lane >> 5;
 
     // DMA instruction j of this wave moves slab rows j * 4 * RPI + rl, chunk
     // (lane % CPR) ^ f(row); f only depends on the low row bits, which j does
-    // not change, so the per-lane part of the address is one 32-bit offset
-    // and the row base is uniform.
+    // not change, so the per-lane part of the address is one 32-bit byte
+    // offset (a VGPR, fixed for the tile) and the rest -- slab, row block j --
+    // a scalar base: each DMA is a few scalar instructions and one load.
+    // Slab kc lies in strip slot kc / spb (spb = Bblk / BK, a power of two).
     const int rl = wave * RPI + lane / CPR;
     const int ce = ((lane % CPR) ^ swz<CPR>(rl)) * 8;  // element offset in the row
-    auto dma_slab = [&](int k0, int buf) {
-        const int64_t ks = k0 / Bblk, ko = k0 % Bblk;
-        const T* lb = L + ks * l_slot + ko;
-        const T* tb = Tt + ks * t_slot + ko;
+    const uint32_t voffA = (uint32_t)((rl * (int)l_ld + ce) * (int)sizeof(T));
+    const uint32_t voffB = (uint32_t)((rl * (int)t_ld + ce) * (int)sizeof(T));
+    const int spb_sh = __builtin_ctz((unsigned)(Bblk / BK)), spb_m = Bblk / BK - 1;
+    auto dma_slab = [&](int kcs, int buf) {
+        const int64_t ks = kcs >> spb_sh, ko = (int64_t)(kcs & spb_m) * BK;
+        const char* lb = reinterpret_cast<const char*>(L + ks * l_slot + ko + (int64_t)m0 * l_ld);
+        const char* tb = reinterpret_cast<const char*>(Tt + ks * t_slot + ko + (int64_t)n0 * t_ld);
 #pragma unroll
-        for (int j = 0; j < IA; ++j) {
-            const int r0 = m0 + j * WAVES * RPI;
+        for (int j = 0; j < IA; ++j)
             __builtin_amdgcn_global_load_lds(
-                reinterpret_cast<const unsigned*>(lb + (int64_t)r0 * l_ld + rl * (int)l_ld + ce),
+                reinterpret_cast<const unsigned*>(lb + (int64_t)j * WAVES * RPI * l_ld * (int64_t)sizeof(T) + voffA),
                 (__attribute__((address_space(3))) void*)(&As[buf][(j * WAVES + wave) * 64]), 16,
                 0, 0);
-        }
 #pragma unroll
-        for (int j = 0; j < IB; ++j) {
-            const int r0 = n0 + j * WAVES * RPI;
+        for (int j = 0; j < IB; ++j)
             __builtin_amdgcn_global_load_lds(
-                reinterpret_cast<const unsigned*>(tb + (int64_t)r0 * t_ld + rl * (int)t_ld + ce),
+                reinterpret_cast<const unsigned*>(tb + (int64_t)j * WAVES * RPI * t_ld * (int64_t)sizeof(T) + voffB),
                 (__attribute__((address_space(3))) void*)(&Bs[buf][(j * WAVES + wave) * 64]), 16,
                 0, 0);
-        }
     };
 
     // L2 prefetch of the slab NBUF slabs ahead: one 4-byte DMA per 128-byte
@@ -328,9 +347,8 @@ __global__ __launch_bounds__((TBM / 128) * (TBN / 128) * 64, 1) void update_lpw_
     // counts in vmcnt, which the waits below allow for.
     static_assert(TBM == 256 && TBN == 256, "the prefetch mapping assumes 256 x 256 tiles");
     __shared__ __attribute__((aligned(16))) unsigned pf_sink[4][2][64];
-    const bool pf_on = prefetch != 0;
-    auto prefetch_slab = [&](int k0) {
-        const int64_t ks = k0 / Bblk, ko = k0 % Bblk;
+    auto prefetch_slab = [&](int kcs) {
+        const int64_t ks = kcs >> spb_sh, ko = (int64_t)(kcs & spb_m) * BK;
         const int r = (wave & 1) * 128 + lane;
         const T* row = wave < 2 ? L + ks * l_slot + ko + (int64_t)(m0 + r) * l_ld
                                 : Tt + ks * t_slot + ko + (int64_t)(n0 + r) * t_ld;
@@ -348,7 +366,7 @@ __global__ __launch_bounds__((TBM / 128) * (TBN / 128) * 64, 1) void update_lpw_
     // the first NBUF - 1 slabs go out before the loads of C
 #pragma unroll
     for (int p = 0; p + 1 < NBUF; ++p)
-        if (p < nk) dma_slab(p * BK, p);
+        if (p < nk) dma_slab(p, p);
 
     // The MFMAs take T as their A operand and L as their B operand, so each
     // 32 x 32 accumulator block holds a block of C transposed: register
@@ -403,11 +421,16 @@ __global__ __launch_bounds__((TBM / 128) * (TBN / 128) * 64, 1) void update_lpw_
     read_ops(0, 0, 0);
     for (int kc = 0; kc < nk; ++kc) {
         const int buf = kc % NBUF;
-        // the buffer of slab kc - 1: every wave finished reading it before the
-        // barrier that ended that slab
-        if (kc + NBUF - 1 < nk) dma_slab((kc + NBUF - 1) * BK, (kc + NBUF - 1) % NBUF);
-        const bool pf_now = pf_on && kc + NBUF < nk;
-        if (pf_now) prefetch_slab((kc + NBUF) * BK);
+        // DMA of slab kc + NBUF - 1 into the buffer of slab kc - 1 (every
+        // wave finished reading it before the barrier that ended that slab)
+        // and the L2 prefetch of slab kc + NBUF (clamped to the last slab).
+        // (In a block of its own: issued in the same block as the LDS reads,
+        // the DMA makes the compiler wait for it before the first read.)
+        const bool dma_now = kc + NBUF - 1 < nk;
+        if (dma_now) {
+            dma_slab(kc + NBUF - 1, (kc + NBUF - 1) % NBUF);
+            prefetch_slab(min(kc + NBUF, nk - 1));
+        }
 #pragma unroll
         for (int ks = 0; ks + 1 < KS; ++ks) {
             read_ops(buf, ks + 1, (ks & 1) ^ 1);
@@ -434,14 +457,10 @@ __global__ __launch_bounds__((TBM / 128) * (TBN / 128) * 64, 1) void update_lpw_
             // this slab's prefetch (issued after them) still in flight -- and
             // this wave's LDS reads of slab kc are done; then a bare barrier
             // (no fence: that would also wait for the prefetch)
-            if (kc + NBUF - 1 < nk) {
-                if (pf_now)
-                    __builtin_amdgcn_s_waitcnt(vmcnt_le((NBUF - 2) * (IA + IB) + PFN, 0));
-                else
-                    __builtin_amdgcn_s_waitcnt(vmcnt_le((NBUF - 2) * (IA + IB), 0));
-            } else {
+            if (dma_now)
+                __builtin_amdgcn_s_waitcnt(vmcnt_le((NBUF - 2) * (IA + IB) + PFN, 0));
+            else
                 __builtin_amdgcn_s_waitcnt(0);
-            }
             __builtin_amdgcn_s_barrier();
             read_ops((kc + 1) % NBUF, 0, KS & 1);
         }
@@ -462,6 +481,7 @@ __global__ __launch_bounds__((TBM / 128) * (TBN / 128) * 64, 1) void update_lpw_
                 *reinterpret_cast<f32x4*>(p + j * 32 + 8 * q) = v;
             }
     }
+    }  // tiles
 }
 
 // The wide-tile kernel is the 16-bit update; HPL_LP_TILES=128 selects the
@@ -480,16 +500,6 @@ int lp_wide_slab() {
     return k;
 }
 
-// L2 prefetch of the wide kernel's slabs (HPL_LP_PREFETCH=0 turns it off,
-// for measurements).
-int lp_prefetch() {
-    static const int p = [] {
-        const char* e = getenv("HPL_LP_PREFETCH");
-        return e && atoi(e) == 0 ? 0 : 1;
-    }();
-    return p;
-}
-
 bool lp_wide() {
     static const int w = [] {
         const char* e = getenv("HPL_LP_TILES");
@@ -501,7 +511,7 @@ bool lp_wide() {
 template <typename T>
 hipError_t launch_t(float* C, int64_t ldc, const void* L, int64_t l_ld, int64_t l_slot,
                     const void* Tt, int64_t t_ld, int64_t t_slot, int M, int N, int K, int Bblk,
-                    hipStream_t s, LpRegion r2) {
+                    hipStream_t s, LpRegion r2, int max_groups) {
     const float* scL = nullptr;
     const float* scT = nullptr;
     if (!order_regions(C, L, Tt, scL, scT, M, N, r2)) return hipSuccess;
@@ -510,17 +520,25 @@ hipError_t launch_t(float* C, int64_t ldc, const void* L, int64_t l_ld, int64_t 
     // the wide kernel moves C as 16-byte row vectors
     const bool c_vec = (ldc % 4) == 0 && (reinterpret_cast<uintptr_t>(C) & 15) == 0 &&
                        (reinterpret_cast<uintptr_t>(r2.C) & 15) == 0;
-    if (lp_wide() && c_vec && M % W == 0 && N % W == 0 && r2.M % W == 0 && r2.N % W == 0) {
-        const int tiles = ((M + W - 1) / W) * ((N + W - 1) / W) +
-                          (r2.M > 0 ? ((r2.M + W - 1) / W) * ((r2.N + W - 1) / W) : 0);
+    // Regions of fewer than 128 wide tiles (half the CUs) take the 128 x 128
+    // tiles: four times the workgroups, so a small update on the critical
+    // path (a diagonal block, the last row and column blocks) ends sooner.
+    // Both kernels give the same bits.
+    const int wide_tiles = (M / W) * (N / W) + (r2.M > 0 ? (r2.M / W) * (r2.N / W) : 0);
+    if (lp_wide() && c_vec && M % W == 0 && N % W == 0 && r2.M % W == 0 && r2.N % W == 0 &&
+        wide_tiles >= 128 && (Bblk & (Bblk - 1)) == 0) {
+        // at most max_groups persistent workgroups (rounded down to a
+        // multiple of 8), which leaves the other CUs to concurrent launches
+        int tiles = wide_tiles;
+        if (max_groups > 0 && max_groups < tiles) tiles = std::max(8, max_groups & ~7);
         if (lp_wide_slab() == 64)
             update_lpw_kernel<T, W, W, 64, 2><<<tiles, (W / 128) * (W / 128) * 64, 0, s>>>(
                 C, ldc, static_cast<const T*>(L), l_ld, l_slot, static_cast<const T*>(Tt), t_ld,
-                t_slot, M, N, K, Bblk, r2, lp_prefetch());
+                t_slot, M, N, K, Bblk, r2);
         else
             update_lpw_kernel<T, W, W, 32, 4><<<tiles, (W / 128) * (W / 128) * 64, 0, s>>>(
                 C, ldc, static_cast<const T*>(L), l_ld, l_slot, static_cast<const T*>(Tt), t_ld,
-                t_slot, M, N, K, Bblk, r2, lp_prefetch());
+                t_slot, M, N, K, Bblk, r2);
         return hipGetLastError();
     }
     const int tiles = region_tiles(M, N) + region_tiles(r2.M, r2.N);
@@ -691,12 +709,14 @@ hipError_t launch_gemm_update_fp8(float* C, int64_t ldc, const void* L, int64_t 
 hipError_t launch_gemm_update_lp(int prec, float* C, int64_t ldc, const void* L, int64_t l_ld,
                                  int64_t l_slot, const void* Tt, int64_t t_ld, int64_t t_slot,
                                  int M, int N, int K, int Bblk, hipStream_t s,
-                                 const LpRegion2* r2in) {
+                                 const LpRegion2* r2in, int max_groups) {
     LpRegion r2{nullptr, nullptr, nullptr, nullptr, nullptr, 0, 0};
     if (r2in) r2 = LpRegion{r2in->C, r2in->L, r2in->T, nullptr, nullptr, r2in->M, r2in->N};
     if (prec == 1)
-        return launch_t<__bf16>(C, ldc, L, l_ld, l_slot, Tt, t_ld, t_slot, M, N, K, Bblk, s, r2);
+        return launch_t<__bf16>(C, ldc, L, l_ld, l_slot, Tt, t_ld, t_slot, M, N, K, Bblk, s, r2,
+                                max_groups);
     if (prec == 2)
-        return launch_t<_Float16>(C, ldc, L, l_ld, l_slot, Tt, t_ld, t_slot, M, N, K, Bblk, s, r2);
+        return launch_t<_Float16>(C, ldc, L, l_ld, l_slot, Tt, t_ld, t_slot, M, N, K, Bblk, s, r2,
+                                  max_groups);
     return hipErrorInvalidValue;
 }

@@ -21,6 +21,22 @@ hipError_t launch_trsm(float* A, int64_t lda, int64_t off, int64_t m, int B, con
                        void* lpT = nullptr, void* lpL = nullptr, int64_t ldlp = 0,
                        float* scT = nullptr, float* scL = nullptr);
 
+// Strip solves of the BF16 factorisation through the diagonal block's full
+// inverses (B <= 256): launch_tinv_prepare turns the block at (off, off) --
+// factorised, with its 32 x 32 inverses in inv (launch_prep) -- into both
+// inverses in `tinv` (tinv_floats(B) floats: OutL then OutU as
+// launch_block_inverse writes them, followed by their MFMA operand copies);
+// launch_trsm_inv then solves both strips as one product each, y = G v, on
+// BF16 MFMA with the operands split into high and low BF16 halves (three
+// products: about 2^-17 relative), writing the FP32 strips and their
+// k-contiguous 16-bit copies like launch_trsm (lp_prec 1: BF16; 2: FP16).
+int64_t tinv_floats(int B);
+hipError_t launch_tinv_prepare(const float* A, int64_t lda, int64_t off, int B, const float* inv,
+                               float* tinv, hipStream_t s);
+hipError_t launch_trsm_inv(float* A, int64_t lda, int64_t off, int64_t m, int B,
+                           const float* tinv, hipStream_t s, int lp_prec, void* lpT, void* lpL,
+                           int64_t ldlp);
+
 // the top strip alone (rows off..off+B-1, columns off+B..off+B+m-1), FP32, B <= 256
 hipError_t launch_trsm_top(float* A, int64_t lda, int64_t off, int64_t m, int B, const float* DT,
                            const float* inv, hipStream_t s);
@@ -46,11 +62,13 @@ struct LpRegion2 {
     int M, N;
 };
 
-// mixed-precision trailing update (gemm_lp.hip): prec 1 = BF16, 2 = FP16
+// mixed-precision trailing update (gemm_lp.hip): prec 1 = BF16, 2 = FP16.
+// max_groups > 0 caps the workgroups of the 256 x 256-tile kernel (one per CU,
+// persistent over the tiles), leaving the other CUs to concurrent launches.
 hipError_t launch_gemm_update_lp(int prec, float* C, int64_t ldc, const void* L, int64_t l_ld,
                                  int64_t l_slot, const void* Tt, int64_t t_ld, int64_t t_slot,
                                  int M, int N, int K, int Bblk, hipStream_t s,
-                                 const LpRegion2* r2 = nullptr);
+                                 const LpRegion2* r2 = nullptr, int max_groups = 0);
 
 // FP8 trailing update (gemm_lp.hip): C[r][c] += sum over the K / Bblk steps p
 // of scL[p][r/32] * scT[p][c/32] * sum_k L[r][k] T[c][k]  (E4M3 codes, FP32

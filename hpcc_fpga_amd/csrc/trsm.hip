@@ -502,6 +502,150 @@ __global__ __launch_bounds__(64) void block_inverse_kernel(const float* __restri
     }
 }
 
+
+// ----------------------------------------------------------------------------
+// Strip solves of the BF16 factorisation as products with the diagonal
+// block's full inverses.  The top and left recurrences above are linear maps
+// y = G v with G lower triangular (G_top = OutU, G_left = OutL transposed, as
+// block_inverse_kernel writes them), so each strip is one product over
+// independent 32 x 32 blocks, y_q = sum_{p <= q} G_qp v_p: no chain from block
+// to block, every load of v issued before the first MFMA.  The product runs on
+// v_mfma_f32_32x32x16_bf16 with each operand split as x = hi + lo (hi =
+// bf16(x), lo = bf16(x - hi), both exact to about 2^-17 |x|) and
+// G v = G_hi v_hi + G_hi v_lo + G_lo v_hi.
+//
+// Operand layout of one 32 x 32 block product (two MFMAs of K = 16): lane
+// (i, h) = (l % 32, l / 32) supplies G[i][16 kb + 8 h + e] and, for vector
+// i, v[16 kb + 8 h + e], e = 0..7.  tinv_pack_kernel writes G in that layout
+// once per step: block (q, p), p <= q, at index q (q + 1) / 2 + p, 2048
+// halves each -- [kb][hi, lo][lane][8].
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ void split_bf16(const float (&x)[8], bf16x8_t& hi, bf16x8_t& lo) {
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const __bf16 h = static_cast<__bf16>(x[e]);
+        hi[e] = h;
+        lo[e] = static_cast<__bf16>(x[e] - static_cast<float>(h));
+    }
+}
+
+template <int B>
+__global__ __launch_bounds__(64) void tinv_pack_kernel(const float* __restrict__ binv,
+                                                       bf16x8_t* __restrict__ pk) {
+    constexpr int NQ = B / 32, NB = NQ * (NQ + 1) / 2;
+    const int bi = blockIdx.x;
+    int q = 0;
+    while ((q + 1) * (q + 2) / 2 <= bi) ++q;
+    const int p = bi - q * (q + 1) / 2;
+    const bool top = blockIdx.y == 0;
+    const int lane = threadIdx.x, i = lane & 31, h = lane >> 5;
+    const float* OutL = binv;
+    const float* OutU = binv + B * B;
+    bf16x8_t* dst = pk + (int64_t)((top ? 0 : NB) + bi) * 256;
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb) {
+        float x[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int k = p * 32 + 16 * kb + 8 * h + e, r = q * 32 + i;
+            x[e] = top ? OutU[r * B + k] : OutL[k * B + r];
+        }
+        bf16x8_t hi, lo;
+        split_bf16(x, hi, lo);
+        dst[(kb * 2 + 0) * 64 + lane] = hi;
+        dst[(kb * 2 + 1) * 64 + lane] = lo;
+    }
+}
+
+// One wave per 32 vectors: grid.x = ngroups_top top groups, then as many left
+// groups.
+template <int B, typename LP>
+__global__ __launch_bounds__(64) void trsm_inv_kernel(float* __restrict__ A, int64_t lda,
+                                                      int64_t off,
+                                                      const bf16x8_t* __restrict__ pk,
+                                                      int ngroups_top, LP* __restrict__ lpT,
+                                                      LP* __restrict__ lpL, int64_t ldlp) {
+    constexpr int NQ = B / 32, NB = NQ * (NQ + 1) / 2;
+    const int lane = threadIdx.x;
+    const int lr = lane & 31, lh = lane >> 5;
+    const bool top = (int)blockIdx.x < ngroups_top;
+    const int64_t v0 = (int64_t)(top ? blockIdx.x : blockIdx.x - ngroups_top) * 32;
+    float* stripT = A + off * lda + off + B;
+    float* stripL = A + (off + B) * lda + off;
+    const bf16x8_t* G = pk + (top ? 0 : NB * 256) + lane;
+
+    // v of vector v0 + lr, k = 32 p + 16 kb + 8 lh + e, split into halves
+    bf16x8_t vh[NQ][2], vl[NQ][2];
+    {
+        float x[NQ][2][8];
+#pragma unroll
+        for (int p = 0; p < NQ; ++p)
+#pragma unroll
+            for (int kb = 0; kb < 2; ++kb) {
+                const int k0 = p * 32 + 16 * kb + 8 * lh;
+                if (top) {
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) x[p][kb][e] = stripT[(int64_t)(k0 + e) * lda + v0 + lr];
+                } else {
+                    const float* src = stripL + (v0 + lr) * lda + k0;
+                    const f32x4 a = *reinterpret_cast<const f32x4*>(src);
+                    const f32x4 b = *reinterpret_cast<const f32x4*>(src + 4);
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        x[p][kb][e] = a[e];
+                        x[p][kb][4 + e] = b[e];
+                    }
+                }
+            }
+#pragma unroll
+        for (int p = 0; p < NQ; ++p)
+#pragma unroll
+            for (int kb = 0; kb < 2; ++kb) split_bf16(x[p][kb], vh[p][kb], vl[p][kb]);
+    }
+
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        f32x16 acc = {0};
+#pragma unroll
+        for (int p = 0; p <= q; ++p) {
+            const bf16x8_t* g = G + (int64_t)(q * (q + 1) / 2 + p) * 256;
+#pragma unroll
+            for (int kb = 0; kb < 2; ++kb) {
+                const bf16x8_t ghi = g[(kb * 2 + 0) * 64], glo = g[(kb * 2 + 1) * 64];
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(glo, vh[p][kb], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ghi, vl[p][kb], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ghi, vh[p][kb], acc, 0, 0, 0);
+            }
+        }
+        // register r: element i = q * 32 + acc_row(r) of vector v0 + lr
+        LP* dst = (top ? lpT : lpL) + (v0 + lr) * ldlp + q * 32 + 4 * lh;
+#pragma unroll
+        for (int g4 = 0; g4 < 4; ++g4) {
+            struct alignas(8) P4 {
+                LP v[4];
+            } p4;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) p4.v[e] = static_cast<LP>(acc[4 * g4 + e]);
+            *reinterpret_cast<P4*>(dst + 8 * g4) = p4;
+        }
+        if (top) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r)
+                stripT[(int64_t)(q * 32 + acc_row(r, lane)) * lda + v0 + lr] = acc[r];
+        } else {
+            float* row = stripL + (v0 + lr) * lda + q * 32 + 4 * lh;
+#pragma unroll
+            for (int g4 = 0; g4 < 4; ++g4) {
+                f32x4 v;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) v[e] = acc[4 * g4 + e];
+                *reinterpret_cast<f32x4*>(row + 8 * g4) = v;
+            }
+        }
+    }
+}
+
 }  // namespace
 
 hipError_t launch_block_inverse(const float* A, int64_t lda, int64_t n, int B, const float* inv,
@@ -608,4 +752,66 @@ hipError_t launch_trsm(float* A, int64_t lda, int64_t off, int64_t m, int B, con
         default:
             return hipErrorInvalidValue;
     }
+}
+
+int64_t tinv_floats(int B) {
+    const int NQ = B / 32;
+    // OutL and OutU, then the packed halves (2048 per block, 2 bytes each)
+    return 2 * (int64_t)B * B + (int64_t)NQ * (NQ + 1) * 2048 / 2;
+}
+
+hipError_t launch_tinv_prepare(const float* A, int64_t lda, int64_t off, int B, const float* inv,
+                               float* tinv, hipStream_t s) {
+    if (B % 32 || B > 256 || B < 32 || (reinterpret_cast<uintptr_t>(tinv) & 15))
+        return hipErrorInvalidValue;
+    const int NQ = B / 32;
+    const float* D = A + off * lda + off;
+    bf16x8_t* pk = reinterpret_cast<bf16x8_t*>(tinv + 2 * (int64_t)B * B);
+    dim3 g1(NQ, 2, 1), g2(NQ * (NQ + 1) / 2, 2);
+    switch (B) {
+#define HPL_TINV(BB)                                                                     \
+    case BB:                                                                             \
+        block_inverse_kernel<BB><<<g1, 64, 0, s>>>(D, lda, inv, 0, tinv);                \
+        tinv_pack_kernel<BB><<<g2, 64, 0, s>>>(tinv, pk);                                \
+        break;
+        HPL_TINV(32)
+        HPL_TINV(64)
+        HPL_TINV(128)
+        HPL_TINV(256)
+#undef HPL_TINV
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_trsm_inv(float* A, int64_t lda, int64_t off, int64_t m, int B,
+                           const float* tinv, hipStream_t s, int lp_prec, void* lpT, void* lpL,
+                           int64_t ldlp) {
+    if (m <= 0) return hipSuccess;
+    if (m % 32 || (lda & 3) || (off & 3) || (ldlp & 3) || (lp_prec != 1 && lp_prec != 2))
+        return hipErrorInvalidValue;
+    const int groups = (int)(m / 32);
+    const bf16x8_t* pk = reinterpret_cast<const bf16x8_t*>(tinv + 2 * (int64_t)B * B);
+    dim3 grid(2 * groups);
+#define HPL_TRSM_INV(BB)                                                                      \
+    case BB:                                                                                  \
+        if (lp_prec == 1)                                                                     \
+            trsm_inv_kernel<BB, __bf16><<<grid, 64, 0, s>>>(A, lda, off, pk, groups,           \
+                                                            static_cast<__bf16*>(lpT),         \
+                                                            static_cast<__bf16*>(lpL), ldlp);  \
+        else                                                                                  \
+            trsm_inv_kernel<BB, _Float16><<<grid, 64, 0, s>>>(A, lda, off, pk, groups,         \
+                                                              static_cast<_Float16*>(lpT),     \
+                                                              static_cast<_Float16*>(lpL),     \
+                                                              ldlp);                           \
+        break;
+    switch (B) {
+        HPL_TRSM_INV(32)
+        HPL_TRSM_INV(64)
+        HPL_TRSM_INV(128)
+        HPL_TRSM_INV(256)
+        default: return hipErrorInvalidValue;
+    }
+#undef HPL_TRSM_INV
+    return hipGetLastError();
 }

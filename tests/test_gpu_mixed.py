@@ -37,6 +37,47 @@ def test_lowprec_factor_close_to_fp32(ctx, prec, tol):
     assert err <= tol, err
 
 
+@pytest.mark.parametrize("prec", ["bf16"])
+def test_strip_solve_by_inverse(ctx, prec):
+    """The BF16 factorisation solves the strips as products with the
+    diagonal block's full inverse, split into high and low BF16 halves (about
+    2^-17 relative).  Step 0's strips see no earlier update, so they equal the
+    oracle's first-panel strips to that accuracy (FP32 recurrence: ~1e-7); a
+    wrong operand layout or a dropped half gives 1e-3 or worse.  The 32 x 32
+    recurrence (HPL_STRIP_SOLVE=recurrence) must meet the same bar."""
+    import os
+
+    from hpcc_fpga_amd import Context
+
+    n, B = 1024, 256
+    print(f"strip solve by inverse {prec}", flush=True)
+    A, _, _, _ = O.generate_sized(n, 5, 0)
+    F_ref = O.gefa_nopvt(A)
+    ref = np.concatenate([F_ref[:B, B:].ravel(), F_ref[B:, :B].ravel()])
+    scale = np.max(np.abs(ref))
+    errs = {}
+    for mode in ("inverse", "recurrence"):
+        if mode == "recurrence":
+            os.environ["HPL_STRIP_SOLVE"] = "recurrence"
+        try:
+            c = ctx if mode == "inverse" else Context(0)
+        finally:
+            os.environ.pop("HPL_STRIP_SOLVE", None)
+        try:
+            c.keep_original(False)
+            c.upload(A)
+            c.gefa_dev(B, prec)
+            F, _ = c.download(matrix=True, solution=False)
+        finally:
+            if c is not ctx:
+                c.close()
+        got = np.concatenate([F[:B, B:].ravel(), F[B:, :B].ravel()])
+        errs[mode] = float(np.max(np.abs(got - ref)) / scale)
+    print(f"  first-panel strips vs oracle: {errs}", flush=True)
+    assert errs["inverse"] <= 5e-5, errs
+    assert errs["recurrence"] <= 5e-6, errs
+
+
 @pytest.mark.parametrize("prec,n,B", [("bf16", 512, 64), ("fp16", 512, 64), ("fp8", 512, 64),
                                       ("fp8", 768, 128), ("bf16", 1024, 256), ("fp16", 1536, 256)])
 def test_lowprec_factor_matches_quantised_emulation(ctx, prec, n, B):
