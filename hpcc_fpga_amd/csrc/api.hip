@@ -725,11 +725,11 @@ hpl_status hpl_create(int device, hpl_ctx** out) {
         hipSuccess)
         c->num_cu = 0;
     if (const char* e = getenv("HPL_TAIL_M")) c->tail_m = std::max(0LL, atoll(e));
-    if (const char* e = getenv("HPL_TAIL_DEPTH")) c->tail_depth = std::max(0, std::min(16, atoi(e)));
+    if (const char* e = getenv("HPL_TAIL_DEPTH")) c->tail_depth = std::max(0, std::min(32, atoi(e)));
     if (const char* e = getenv("HPL_PANEL_SPIN_LIMIT")) c->pv_spin_limit = atol(e);
     if (const char* e = getenv("HPL_F64_DEPTH")) c->depth_f64 = std::max(1, std::min(16, atoi(e)));
     if (const char* e = getenv("HPL_UPDATE_DEPTH"))
-        c->depth_fp32 = c->depth_lp = std::max(1, std::min(16, atoi(e)));
+        c->depth_fp32 = c->depth_lp = std::max(1, std::min(32, atoi(e)));
     if (hplx::init_streams(c) != HPL_SUCCESS) {
         hpl_destroy(c);
         return HPL_ERR_HIP;

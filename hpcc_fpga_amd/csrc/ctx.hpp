@@ -55,7 +55,7 @@ struct hpl_ctx {
     // element per pass), which `depth` divides: 12 (n = 65536, BF16: 327 ms
     // against 334 at 8 and 329 at 16; FP8: 299 against 314 at 8).
     int depth = 1;
-    int depth_fp32 = 4, depth_lp = 12;
+    int depth_fp32 = 4, depth_lp = 16;
     std::string err;
 
     // resident system

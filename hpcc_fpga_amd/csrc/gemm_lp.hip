@@ -249,6 +249,41 @@ __device__ __forceinline__ int swz(int r) {
     else return ((r >> 1) ^ (r >> 2)) & 3;
 }
 
+// A raw buffer resource over the bytes from p on (stride 0, range 2^31 - 1;
+// dword 3 as for every raw gfx9 buffer: 32-bit data format).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t raw_rsrc(const void* p) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, 0x7fffffff,
+                                             0x00020000);
+}
+
+// N 16-byte-per-lane loads into LDS from rows of a raw buffer at `base`:
+// load j reads byte offset voff + j * jst of each lane and fills the 64
+// 16-byte LDS chunks at lds + j * lds_st.  (Buffer resources live in plain
+// device functions: held in a lambda of a kernel template, they cost the
+// kernel its host-side launch stub.)
+template <int N>
+__device__ __forceinline__ void dma_rows(const void* base, uint32_t voff, uint32_t jst,
+                                         u32x4* lds, int lds_st) {
+    const __amdgpu_buffer_rsrc_t r = raw_rsrc(base);
+#pragma unroll
+    for (int j = 0; j < N; ++j)
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(
+            r, (__attribute__((address_space(3))) void*)(lds + j * lds_st), 16, voff, j * jst, 0,
+            0);
+}
+
+// Two 4-byte-per-lane loads into the LDS words sink[0..63] and sink[64..127]
+// (read by nobody): they bring the 128-byte lines at voff and voff + st of
+// each lane into L2.
+__device__ __forceinline__ void prefetch_rows(const void* base, uint32_t voff, uint32_t st,
+                                              unsigned* sink) {
+    const __amdgpu_buffer_rsrc_t r = raw_rsrc(base);
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (__attribute__((address_space(3))) void*)sink, 4,
+                                             voff, 0, 0, 0);
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(
+        r, (__attribute__((address_space(3))) void*)(sink + 64), 4, voff, st, 0, 0);
+}
+
 // s_waitcnt immediate that waits until at most n vector memory operations and
 // at most lgkm LDS / scalar memory operations of this wave are outstanding
 // (expcnt not waited for)
@@ -320,23 +355,23 @@ __global__ __launch_bounds__((TBM / 128) * (TBN / 128) * 64, 1) void update_lpw_
     const int ce = ((lane % CPR) ^ swz<CPR>(rl)) * 8;  // element offset in the row
     const uint32_t voffA = (uint32_t)((rl * (int)l_ld + ce) * (int)sizeof(T));
     const uint32_t voffB = (uint32_t)((rl * (int)t_ld + ce) * (int)sizeof(T));
+    const uint32_t jstA = (uint32_t)(WAVES * RPI * l_ld * (int64_t)sizeof(T));  // row block j
+    const uint32_t jstB = (uint32_t)(WAVES * RPI * t_ld * (int64_t)sizeof(T));
     const int spb_sh = __builtin_ctz((unsigned)(Bblk / BK)), spb_m = Bblk / BK - 1;
-    auto dma_slab = [&](int kcs, int buf) {
+    // The slab's rows as raw buffers (base = the slab's first row of the
+    // tile): buffer loads into LDS take the base and the row block's offset in
+    // scalar registers and the lane's offset in one VGPR.  (Unlike the FLAT
+    // form they count in vmcnt only, so the compiler can wait for just the
+    // LDS reads a k-step needs.)
+    auto slab_base = [&](const T* base, int kcs, int64_t slot, int64_t ld, int r0) {
         const int64_t ks = kcs >> spb_sh, ko = (int64_t)(kcs & spb_m) * BK;
-        const char* lb = reinterpret_cast<const char*>(L + ks * l_slot + ko + (int64_t)m0 * l_ld);
-        const char* tb = reinterpret_cast<const char*>(Tt + ks * t_slot + ko + (int64_t)n0 * t_ld);
-#pragma unroll
-        for (int j = 0; j < IA; ++j)
-            __builtin_amdgcn_global_load_lds(
-                reinterpret_cast<const unsigned*>(lb + (int64_t)j * WAVES * RPI * l_ld * (int64_t)sizeof(T) + voffA),
-                (__attribute__((address_space(3))) void*)(&As[buf][(j * WAVES + wave) * 64]), 16,
-                0, 0);
-#pragma unroll
-        for (int j = 0; j < IB; ++j)
-            __builtin_amdgcn_global_load_lds(
-                reinterpret_cast<const unsigned*>(tb + (int64_t)j * WAVES * RPI * t_ld * (int64_t)sizeof(T) + voffB),
-                (__attribute__((address_space(3))) void*)(&Bs[buf][(j * WAVES + wave) * 64]), 16,
-                0, 0);
+        return base + ks * slot + ko + (int64_t)r0 * ld;
+    };
+    auto dma_slab = [&](int kcs, int buf) {
+        dma_rows<IA>(slab_base(L, kcs, l_slot, l_ld, m0), voffA, jstA, &As[buf][wave * 64],
+                     WAVES * 64);
+        dma_rows<IB>(slab_base(Tt, kcs, t_slot, t_ld, n0), voffB, jstB, &Bs[buf][wave * 64],
+                     WAVES * 64);
     };
 
     // L2 prefetch of the slab NBUF slabs ahead: one 4-byte DMA per 128-byte
@@ -347,18 +382,13 @@ __global__ __launch_bounds__((TBM / 128) * (TBN / 128) * 64, 1) void update_lpw_
     // counts in vmcnt, which the waits below allow for.
     static_assert(TBM == 256 && TBN == 256, "the prefetch mapping assumes 256 x 256 tiles");
     __shared__ __attribute__((aligned(16))) unsigned pf_sink[4][2][64];
+    const int pf_r = (wave & 1) * 128 + lane;  // the lane's first row
+    const uint32_t pf_off = (uint32_t)(pf_r * (wave < 2 ? l_ld : t_ld) * (int64_t)sizeof(T));
+    const uint32_t pf_st = (uint32_t)(64 * (wave < 2 ? l_ld : t_ld) * (int64_t)sizeof(T));
     auto prefetch_slab = [&](int kcs) {
-        const int64_t ks = kcs >> spb_sh, ko = (int64_t)(kcs & spb_m) * BK;
-        const int r = (wave & 1) * 128 + lane;
-        const T* row = wave < 2 ? L + ks * l_slot + ko + (int64_t)(m0 + r) * l_ld
-                                : Tt + ks * t_slot + ko + (int64_t)(n0 + r) * t_ld;
-        const int64_t ld64 = (wave < 2 ? l_ld : t_ld) * 64;  // 64 rows further
-        __builtin_amdgcn_global_load_lds(reinterpret_cast<const unsigned*>(row),
-                                         (__attribute__((address_space(3))) void*)(&pf_sink[wave][0][0]),
-                                         4, 0, 0);
-        __builtin_amdgcn_global_load_lds(reinterpret_cast<const unsigned*>(row + ld64),
-                                         (__attribute__((address_space(3))) void*)(&pf_sink[wave][1][0]),
-                                         4, 0, 0);
+        prefetch_rows(wave < 2 ? slab_base(L, kcs, l_slot, l_ld, m0)
+                               : slab_base(Tt, kcs, t_slot, t_ld, n0),
+                      pf_off, pf_st, &pf_sink[wave][0][0]);
     };
     constexpr int PFN = 2;  // prefetch DMAs per wave and slab
 
