@@ -439,20 +439,43 @@ hpl_status gefa_pipelined(hpl_ctx* c, uint32_t B) {
                 if (r) return r;
                 continue;
             }
-            // The diagonal block's update and factorisation go first, on s2,
+            // Block k still needs steps k0 .. k-1.  For all but a group's
+            // first block, steps k0 .. k-2 were applied one step ahead (on
+            // s3, below), so only step k-1 is left on the critical path.  The
+            // diagonal block's update and factorisation go first, on s2,
             // while s0 updates the rest of the row and column blocks; the
-            // strip solves then wait for both.
-            const int K = (int)((k - k0) * B);
+            // strip solves then wait for both.  (Each element still receives
+            // the steps in order, and an update split over launches gives the
+            // bits of one launch: the accumulators start from C.)
+            const bool ahead = k > bnd[g];
+            const int64_t from = ahead ? k - 1 : k0;
+            const int K = (int)((k - from) * B);
             HIPCHK(c, hipEventRecord(c->evS, c->s0));
             HIPCHK(c, hipStreamWaitEvent(c->s2, c->evS, 0));
-            r = launch_update(c, c->s2, k0, kb, kb, (int)B, (int)B, K, B);
+            if (ahead) HIPCHK(c, hipStreamWaitEvent(c->s2, c->evL, 0));
+            r = launch_update(c, c->s2, from, kb, kb, (int)B, (int)B, K, B);
             if (!r) r = panel_diag(c, c->s2, k, B);
             if (r) return r;
             HIPCHK(c, hipEventRecord(c->evD, c->s2));
+            if (ahead) HIPCHK(c, hipStreamWaitEvent(c->s0, c->evL, 0));
             if (m > (int64_t)B) {
-                r = launch_update2(c, c->s0, k0, UpdRegion{kb, kb + B, (int)B, (int)(m - B)},
+                r = launch_update2(c, c->s0, from, UpdRegion{kb, kb + B, (int)B, (int)(m - B)},
                                    UpdRegion{kb + B, kb, (int)(m - B), (int)B}, K, B);
                 if (r) return r;
+            }
+            // steps k0 .. k-1 of the group's next block, on s3 beside this
+            // step's factorisation (they need only the strips up to k-1)
+            if (k + 1 < bnd[g + 1]) {
+                const int64_t kb1 = kb + B, m1 = m - B;
+                const int K1 = (int)((k - k0) * B);
+                HIPCHK(c, hipStreamWaitEvent(c->s3, c->evS, 0));
+                r = launch_update(c, c->s3, k0, kb1, kb1, (int)B, (int)B, K1, B);
+                if (!r && m1 > (int64_t)B)
+                    r = launch_update2(c, c->s3, k0,
+                                       UpdRegion{kb1, kb1 + B, (int)B, (int)(m1 - B)},
+                                       UpdRegion{kb1 + B, kb1, (int)(m1 - B), (int)B}, K1, B);
+                if (r) return r;
+                HIPCHK(c, hipEventRecord(c->evL, c->s3));
             }
             HIPCHK(c, hipStreamWaitEvent(c->s0, c->evD, 0));
             r = panel_strips(c, c->s0, k, B);
@@ -598,11 +621,13 @@ hpl_status init_streams(hpl_ctx* c) {
     if (hipStreamCreateWithFlags(&c->s0, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&c->s1, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&c->s2, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->s3, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->e0) != hipSuccess || hipEventCreate(&c->e1) != hipSuccess ||
         hipEventCreateWithFlags(&c->evA, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&c->evP, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&c->evS, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c->evD, hipEventDisableTiming) != hipSuccess)
+        hipEventCreateWithFlags(&c->evD, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->evL, hipEventDisableTiming) != hipSuccess)
         return fail(c, HPL_ERR_HIP, "stream or event creation failed");
     return HPL_SUCCESS;
 }
@@ -744,6 +769,7 @@ void hpl_destroy(hpl_ctx* c) {
     if (c->s0) (void)hipStreamSynchronize(c->s0);
     if (c->s1) (void)hipStreamSynchronize(c->s1);
     if (c->s2) (void)hipStreamSynchronize(c->s2);
+    if (c->s3) (void)hipStreamSynchronize(c->s3);
     if (!c->external)
         for (void* p : {(void*)c->dA, (void*)c->dA0, (void*)c->db, (void*)c->dx, (void*)c->db64,
                         (void*)c->dxt, (void*)c->dLt, (void*)c->dDT, (void*)c->dInv,
@@ -765,6 +791,8 @@ void hpl_destroy(hpl_ctx* c) {
     if (c->evP) (void)hipEventDestroy(c->evP);
     if (c->evS) (void)hipEventDestroy(c->evS);
     if (c->evD) (void)hipEventDestroy(c->evD);
+    if (c->evL) (void)hipEventDestroy(c->evL);
+    if (c->s3) (void)hipStreamDestroy(c->s3);
     if (c->s2) (void)hipStreamDestroy(c->s2);
     if (c->s1) (void)hipStreamDestroy(c->s1);
     if (c->s0) (void)hipStreamDestroy(c->s0);
