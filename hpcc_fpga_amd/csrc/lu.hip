@@ -9,12 +9,15 @@
 // the reference loop (no FMA contraction), so the result is bit-identical.
 //
 // The block lives in LDS and is processed in panels of 32 pivot rows:
-//   1. wave 0 eliminates the 32 panel rows (all columns right of the panel
-//      start) with the panel held in registers; pivots and multipliers are
-//      broadcast with v_readlane, so the 32 steps need no barrier;
-//   2. every row below the panel is solved against the panel's 32 x 32
-//      diagonal block (the "left" closed form), one row per thread;
-//   3. the trailing part receives the panel's 32 rank-1 updates in k order,
+//   A. wave 0 factorises the panel's 32 x 32 diagonal sub-block, one column
+//      per lane; each step's pivot and multipliers come from the pivot
+//      column's lane by v_readlane (scalar operands, no LDS round trip), so
+//      the 32 steps need no barrier;
+//   B. every column right of the sub-block (the panel's top strip) and every
+//      row below it (its left strip) is solved against the factorised
+//      sub-block in closed form, one vector per thread, the sub-block's
+//      coefficients read as 16-byte LDS broadcasts from an aligned copy;
+//   C. the trailing part receives the panel's 32 rank-1 updates in k order,
 //      4 x 4 register tiles per thread.
 // Three workgroup barriers per panel instead of one or two per pivot.
 // Blocks up to 128 are handled here; larger diagonal blocks are factorised by
@@ -36,6 +39,10 @@ __global__ __launch_bounds__(NT) void lu_diag_kernel(float* __restrict__ A, int6
 #pragma clang fp contract(off)
     constexpr int LD = B + 1;  // padded LDS row
     __shared__ float Ds[B * LD];
+    // the factorised 32 x 32 sub-block of the current panel, row-major (S)
+    // and transposed (St), rows 16-byte aligned for vector broadcasts
+    __shared__ __attribute__((aligned(16))) float S[32 * 32];
+    __shared__ __attribute__((aligned(16))) float St[32 * 32];
     const int t = threadIdx.x;
     const int lane = t & 63, wave = t >> 6;
     float* base = A + off * lda + off;
@@ -65,67 +72,87 @@ __global__ __launch_bounds__(NT) void lu_diag_kernel(float* __restrict__ A, int6
 
     bool bad = false;
     for (int P0 = 0; P0 < B; P0 += 32) {
-        // ---- 1. panel rows [P0, P0+32), columns [P0, B): wave 0, registers
+        // ---- A. sub-block rows and columns [P0, P0+32): lane c % 32 holds
+        // column P0 + c.  Step j: row j right of the diagonal is scaled by
+        // -1/p; then D[r][c] += D[j][c] * D[r][j] (r, c > j), D[r][j] being
+        // lane j's (final) value of row r.
         if (wave == 0) {
-            const int c0 = P0 + lane, c1 = P0 + 64 + lane;
-            const bool h0 = c0 < B, h1 = c1 < B;
-            float d0[32], d1[32];
+            const int c = lane & 31;
+            float d[32];
 #pragma unroll
-            for (int r = 0; r < 32; ++r) {
-                d0[r] = h0 ? Ds[(P0 + r) * LD + c0] : 0.0f;
-                d1[r] = h1 ? Ds[(P0 + r) * LD + c1] : 0.0f;
-            }
+            for (int r = 0; r < 32; ++r) d[r] = Ds[(P0 + r) * LD + P0 + c];
 #pragma unroll
             for (int j = 0; j < 32; ++j) {
-                const float piv = bcast(d0[j], j);
+                const float piv = bcast(d[j], j);
                 const float inv = -1.0f / piv;
                 bad |= (piv == 0.0f) || !__builtin_isfinite(inv);
-                // scale pivot row j right of the diagonal; diagonal := -1/p
-                d0[j] = (lane > j) ? d0[j] * inv : ((lane == j) ? inv : d0[j]);
-                d1[j] = d1[j] * inv;
-                // the multipliers of column j (lane j's rows) reach every lane
-                // through ds_bpermute: vector results, so the 31 - j
-                // broadcasts pipeline without scalar registers or the
-                // readlane hazards
+                d[j] = (c > j) ? d[j] * inv : ((c == j) ? inv : d[j]);
 #pragma unroll
                 for (int r = j + 1; r < 32; ++r) {
-                    const float s = __shfl(d0[r], j);
-                    const float u = d0[r] + d0[j] * s;
-                    d0[r] = (lane > j) ? u : d0[r];  // a select, not a branch
-                    d1[r] = d1[r] + d1[j] * s;
+                    const float sr = bcast(d[r], j);
+                    const float u = d[r] + d[j] * sr;
+                    d[r] = (c > j) ? u : d[r];  // a select, not a branch
                 }
-                // finish this step's updates here: the compiler would
-                // otherwise delay those of d1 (off the pivot chain) to the
-                // end and keep every broadcast value alive until then
-#pragma unroll
-                for (int r = 0; r < 32; ++r) asm volatile("" : "+v"(d0[r]), "+v"(d1[r]));
             }
+            if (lane < 32) {
 #pragma unroll
-            for (int r = 0; r < 32; ++r) {
-                if (h0) Ds[(P0 + r) * LD + c0] = d0[r];
-                if (h1) Ds[(P0 + r) * LD + c1] = d1[r];
+                for (int r = 0; r < 32; ++r) {
+                    Ds[(P0 + r) * LD + P0 + c] = d[r];
+                    S[r * 32 + c] = d[r];
+                    St[c * 32 + r] = d[r];
+                }
             }
         }
         __syncthreads();
         const int R0 = P0 + 32;  // first row / column after the panel
         const int W = B - R0;    // rows below and columns right of the panel
         if (W == 0) break;
-        // ---- 2. rows below: X[j][P0+i] += X[j][P0+k] * D[P0+k][P0+i], i > k
-        if (t < W) {
-            const int j = R0 + t;
+        // ---- B. top strip (rows P0.., column R0 + t) and left strip (row
+        // R0 + t - W, columns P0..), one vector per thread, in step order
+        for (int v = t; v < 2 * W; v += NT) {
             float x[32];
+            if (v < W) {
+                // top: step j scales x_j by D[j][j] (= -1/p), then
+                // x_r += x_j * D[r][j] for r > j
+                const int cc = R0 + v;
 #pragma unroll
-            for (int i = 0; i < 32; ++i) x[i] = Ds[j * LD + P0 + i];
+                for (int r = 0; r < 32; ++r) x[r] = Ds[(P0 + r) * LD + cc];
 #pragma unroll
-            for (int k = 0; k < 32; ++k)
+                for (int j = 0; j < 32; ++j) {
+                    x[j] = x[j] * S[j * 32 + j];
 #pragma unroll
-                for (int i = k + 1; i < 32; ++i)
-                    x[i] = x[i] + x[k] * Ds[(P0 + k) * LD + P0 + i];
+                    for (int r4 = (j + 1) / 4; r4 < 8; ++r4) {
+                        const float4 sv = *reinterpret_cast<const float4*>(&St[j * 32 + 4 * r4]);
+                        const float s4[4] = {sv.x, sv.y, sv.z, sv.w};
 #pragma unroll
-            for (int i = 0; i < 32; ++i) Ds[j * LD + P0 + i] = x[i];
+                        for (int q = 0; q < 4; ++q)
+                            if (4 * r4 + q > j) x[4 * r4 + q] = x[4 * r4 + q] + x[j] * s4[q];
+                    }
+                }
+#pragma unroll
+                for (int r = 0; r < 32; ++r) Ds[(P0 + r) * LD + cc] = x[r];
+            } else {
+                // left: x_i += x_k * D[k][i] for i > k, k in order
+                const int rr = R0 + v - W;
+#pragma unroll
+                for (int i = 0; i < 32; ++i) x[i] = Ds[rr * LD + P0 + i];
+#pragma unroll
+                for (int k = 0; k < 32; ++k) {
+#pragma unroll
+                    for (int i4 = (k + 1) / 4; i4 < 8; ++i4) {
+                        const float4 sv = *reinterpret_cast<const float4*>(&S[k * 32 + 4 * i4]);
+                        const float s4[4] = {sv.x, sv.y, sv.z, sv.w};
+#pragma unroll
+                        for (int q = 0; q < 4; ++q)
+                            if (4 * i4 + q > k) x[4 * i4 + q] = x[4 * i4 + q] + x[k] * s4[q];
+                    }
+                }
+#pragma unroll
+                for (int i = 0; i < 32; ++i) Ds[rr * LD + P0 + i] = x[i];
+            }
         }
         __syncthreads();
-        // ---- 3. trailing W x W: D[j][i] += D[P0+k][i] * D[j][P0+k], k = 0..31
+        // ---- C. trailing W x W: D[j][i] += D[P0+k][i] * D[j][P0+k], k = 0..31
         const int tiles = (W / 4) * (W / 4);
         for (int tt = t; tt < tiles; tt += NT) {
             const int tj = R0 + (tt / (W / 4)) * 4, ti = R0 + (tt % (W / 4)) * 4;
