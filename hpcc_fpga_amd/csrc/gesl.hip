@@ -412,21 +412,25 @@ hipError_t gesl_sweeps_split(const float* A, int64_t lda, int64_t n, int B, cons
 // belongs to workgroup s mod G, which keeps its running sum z_s in LDS for the
 // whole launch.  Forward sweep, block step q ascending: every slice beyond
 // block q takes block q's contribution z_s += A[qB .. qB+B)[slice]^T y_q as
-// soon as y_q is published; a slice of block q+1 then publishes z_s and, once
-// the T = B / S slices of its block have, computes its part of
-// y_{q+1} = W_L(q+1)^T z_{q+1} from the block's inverse and publishes it.  The
-// backward sweep mirrors it (descending, U and -W_U).  Each workgroup handles
-// its slices in block order, so the slice on the critical path goes first.
+// soon as y_q is known; a slice of block q+1, once complete, publishes its
+// share of y_{q+1} = W_L(q+1)^T z_{q+1}: the product of the slice's S rows of
+// the inverse with z_s, a B-vector.  y_{q+1} is the sum of the T = B / S
+// shares, which every reader forms itself in slice order (so all see the same
+// values): one publication per block on the critical path, not a gather of
+// z followed by a publication of y.  The backward sweep mirrors it
+// (descending, U and -W_U), each slice starting from its entries of y; the
+// owners of a block finally write their entries of x.
 // Publication: values are written with agent-scope coherent stores, every
 // wave drains its stores, then the slice's flag is set (1 forward, 2
 // backward; a small kernel clears the flags before each launch).  A waiter
 // polls the T flags of the block it needs, bounded: past the limit the call
 // fails with status bit 2 and every workgroup leaves.  The factor tile of a
-// contribution and the inverse tile of a diagonal solve do not depend on the
-// awaited values, so they are loaded before the wait and only the flag
-// latency and the arithmetic stay on the critical path.
+// contribution and the inverse rows of a share do not depend on the awaited
+// values, so they are loaded before the wait and only the flag latency and
+// the arithmetic stay on the critical path.
 // ---------------------------------------------------------------------------
 constexpr int kMaxOwn = 16;            // slices per workgroup (LDS capacity)
+constexpr int kMaxT = 4;               // shares per block (B / S, B <= 256)
 constexpr long kGeslSpinLimit = 1L << 22;
 
 __device__ __forceinline__ void pub_f(float* p, float v) {
@@ -456,6 +460,7 @@ __global__ __launch_bounds__(NT) void gesl_persist_kernel(const float* __restric
     constexpr int RG = NT / CG;         // row groups
     constexpr int RPT = B / RG;         // rows of a tile per thread
     static_assert(RPT >= 1 && RPT * RG == B, "tile split");
+    static_assert(B <= NT && T <= kMaxT, "one share column per thread");
     __shared__ float zown[kMaxOwn * S];  // running sums of the owned slices
     __shared__ float vbuf[B];            // the awaited block vector
     __shared__ float red[RG * S];
@@ -464,8 +469,7 @@ __global__ __launch_bounds__(NT) void gesl_persist_kernel(const float* __restric
     const int NS = (int)(n / S), N = (int)(n / B);
     const int own = (NS - g + G - 1) / G;  // slices g, g + G, ...
     const int cg = t % CG, rg = t / CG;
-    unsigned* zflag = ws.flags;
-    unsigned* xflag = ws.flags + NS;
+    unsigned* flag = ws.flags;
 
     for (int k = 0; k < own; ++k)
         for (int c = t; c < S; c += NT) zown[k * S + c] = x[(int64_t)(g + k * G) * S + c];
@@ -477,8 +481,8 @@ __global__ __launch_bounds__(NT) void gesl_persist_kernel(const float* __restric
         for (int i = 0; i < RPT; ++i)
             tile[i] = *reinterpret_cast<const f32x4*>(base + (int64_t)(rg + RG * i) * ld + 4 * cg);
     };
-    // out[c] = (add ? out[c] : 0) + sign * sum_p tile[p][c] vbuf[p]
-    auto gemv = [&](float* out, float sign, bool add) {
+    // out[c] += sign * sum_p tile[p][c] vbuf[p]
+    auto gemv = [&](float* out, float sign) {
         f32x4 a = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
         for (int i = 0; i < RPT; ++i) {
@@ -494,7 +498,7 @@ __global__ __launch_bounds__(NT) void gesl_persist_kernel(const float* __restric
             float sum = 0.0f;
 #pragma unroll 8
             for (int r = 0; r < RG; ++r) sum += red[r * S + t];
-            out[t] = add ? out[t] + sign * sum : sign * sum;
+            out[t] = out[t] + sign * sum;
         }
         __syncthreads();
     };
@@ -520,61 +524,89 @@ __global__ __launch_bounds__(NT) void gesl_persist_kernel(const float* __restric
         __syncthreads();
         return abort_flag != 0;
     };
-    // S values of LDS src to dst, then the slice's flag
-    auto publish = [&](float* dst, const float* src, unsigned* flag, unsigned val) {
-        if (t < S) pub_f(dst + t, src[t]);
+    // block q's vector: the sum of its T shares, in slice order
+    auto fetch_sum = [&](int q, const float* shares, int lo, int len, float* dst) {
+        const float* src = shares + (int64_t)q * T * B;
+        for (int p = t; p < len; p += NT) {
+            float v = 0.0f;
+#pragma unroll
+            for (int j = 0; j < T; ++j) v += get_f(src + j * B + lo + p);
+            dst[p] = v;
+        }
+        __syncthreads();
+    };
+    // slice s (owned slice k, block Q): its share sign * W[slice rows]^T z_s,
+    // published with the slice's flag; the inverse rows are loaded first
+    // (share_load), before the wait for the slice's last contribution
+    float wrow[S];
+    auto share_load = [&](int Q, int s, bool fwd) {
+        const float* w = binv + (int64_t)Q * 2 * B * B + (fwd ? 0 : (int64_t)B * B) +
+                         (int64_t)(s - Q * T) * S * B;
+        if (t < B) {
+#pragma unroll
+            for (int r = 0; r < S; ++r) wrow[r] = w[(int64_t)r * B + t];
+        }
+    };
+    auto share = [&](int k, int s, bool fwd) {
+        const float* z = zown + k * S;
+        float acc = 0.0f;
+        if (t < B) {
+#pragma unroll
+            for (int r = 0; r < S; ++r) acc = fmaf(wrow[r], z[r], acc);
+            pub_f((fwd ? ws.pf : ws.pb) + (int64_t)s * B + t, fwd ? acc : -acc);
+        }
         __builtin_amdgcn_s_waitcnt(0);  // this wave's stores have completed
         __syncthreads();
-        if (t == 0) __hip_atomic_store(flag, val, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    };
-    auto fetch_vec = [&](const float* src) {
-        for (int p = t; p < B; p += NT) vbuf[p] = get_f(src + p);
-        __syncthreads();
-    };
-    // block Q's diagonal solve for owned slice k (= s): forward y_s, backward x_s
-    auto diag = [&](int Q, int k, int s, bool fwd) {
-        float* z = zown + k * S;
-        const unsigned val = fwd ? 1u : 2u;
-        float* zb = fwd ? ws.zf : ws.zb;
-        publish(zb + (int64_t)s * S, z, zflag + s, val);
-        load_tile(binv + (int64_t)Q * 2 * B * B + (fwd ? 0 : (int64_t)B * B) + (s - Q * T) * S, B);
-        if (wait(zflag + (int64_t)Q * T, val)) return true;
-        fetch_vec(zb + (int64_t)Q * B);
-        gemv(z, fwd ? 1.0f : -1.0f, false);
-        publish((fwd ? ws.y : x) + (int64_t)s * S, z, xflag + s, val);
-        return false;
+        if (t == 0)
+            __hip_atomic_store(flag + s, fwd ? 1u : 2u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     };
     // block q's contribution to owned slice k (= s)
     auto contrib = [&](int q, int k, int s, bool fwd) {
         load_tile(A + (int64_t)q * B * lda + (int64_t)s * S, lda);
-        if (wait(xflag + (int64_t)q * T, fwd ? 1u : 2u)) return true;
-        fetch_vec((fwd ? ws.y : x) + (int64_t)q * B);
-        gemv(zown + k * S, fwd ? 1.0f : -1.0f, true);
+        if (wait(flag + (int64_t)q * T, fwd ? 1u : 2u)) return true;
+        fetch_sum(q, fwd ? ws.pf : ws.pb, 0, B, vbuf);
+        gemv(zown + k * S, fwd ? 1.0f : -1.0f);
         return false;
     };
     __syncthreads();
 
     // forward: L y = b
-    if (g < T && diag(0, 0, g, true)) return;
+    if (g < T) {
+        share_load(0, g, true);
+        share(0, g, true);
+    }
     for (int q = 0; q + 1 < N; ++q)
         for (int k = 0; k < own; ++k) {
             const int s = g + k * G, blk = s / T;
             if (blk <= q) continue;
+            if (blk == q + 1) share_load(q + 1, s, true);
             if (contrib(q, k, s, true)) return;
-            if (blk == q + 1 && diag(q + 1, k, s, true)) return;
+            if (blk == q + 1) share(k, s, true);
         }
-    // backward: U x = y (z of each owned slice starts at its y)
+    // backward: U x = y; each owned slice starts at its entries of y,
+    // last block first
     for (int k = own - 1; k >= 0; --k) {
-        const int s = g + k * G;
-        if (s / T == N - 1 && diag(N - 1, k, s, false)) return;
+        const int s = g + k * G, blk = s / T;
+        if (blk == N - 1) share_load(N - 1, s, false);
+        if (wait(flag + (int64_t)blk * T, 1u)) return;
+        fetch_sum(blk, ws.pf, (s - blk * T) * S, S, zown + k * S);
+        if (blk == N - 1) share(k, s, false);
     }
     for (int q = N - 1; q >= 1; --q)
         for (int k = own - 1; k >= 0; --k) {
             const int s = g + k * G, blk = s / T;
             if (blk >= q) continue;
+            if (blk == q - 1) share_load(q - 1, s, false);
             if (contrib(q, k, s, false)) return;
-            if (blk == q - 1 && diag(q - 1, k, s, false)) return;
+            if (blk == q - 1) share(k, s, false);
         }
+    // x: the owners of a block sum its backward shares
+    for (int k = 0; k < own; ++k) {
+        const int s = g + k * G, blk = s / T;
+        if (wait(flag + (int64_t)blk * T, 2u)) return;
+        fetch_sum(blk, ws.pb, (s - blk * T) * S, S, zown + k * S);
+        for (int c = t; c < S; c += NT) x[(int64_t)s * S + c] = zown[k * S + c];
+    }
 }
 
 // Workgroups of the single-launch substitution of order n at block B, or 0
@@ -612,8 +644,8 @@ int64_t persist_groups(int64_t n, int B) {
 hipError_t gesl_persist(const float* A, int64_t lda, int64_t n, int B, const float* binv,
                         float* x, const GeslWs& ws, int64_t G, hipStream_t s) {
     const int64_t NS = n / std::min(64, B);
-    gesl_persist_reset_kernel<<<(unsigned)std::min<int64_t>((2 * NS + 255) / 256, 64), 256, 0,
-                                s>>>(ws.flags, 2 * NS, ws.status);
+    gesl_persist_reset_kernel<<<(unsigned)std::min<int64_t>((NS + 255) / 256, 64), 256, 0,
+                                s>>>(ws.flags, NS, ws.status);
     const unsigned grid = (unsigned)G;
     switch (B) {
         case 32:
@@ -668,17 +700,17 @@ hipError_t launch_gesl(const float* A, int64_t lda, int64_t n, int B, const floa
 int64_t gesl_persist_groups(int64_t n, int B) { return persist_groups(n, B); }
 
 size_t gesl_ws_bytes(int64_t n) {
-    // zf, zb, y (n floats each), two flags per slice of >= 32 entries, status
-    return (size_t)(3 * n) * sizeof(float) + (size_t)(2 * (n / 32) + 64) * sizeof(unsigned);
+    // forward and backward shares (B per slice of S: at most kMaxT n floats
+    // each), one flag per slice of >= 32 entries, status
+    return (size_t)(2 * kMaxT * n) * sizeof(float) + (size_t)(n / 32 + 64) * sizeof(unsigned);
 }
 
 GeslWs gesl_ws_carve(void* base, int64_t n) {
     GeslWs w;
     float* f = static_cast<float*>(base);
-    w.zf = f;
-    w.zb = f + n;
-    w.y = f + 2 * n;
-    w.flags = reinterpret_cast<unsigned*>(f + 3 * n);
-    w.status = reinterpret_cast<int*>(w.flags + 2 * (n / 32));
+    w.pf = f;
+    w.pb = f + kMaxT * n;
+    w.flags = reinterpret_cast<unsigned*>(f + 2 * kMaxT * n);
+    w.status = reinterpret_cast<int*>(w.flags + n / 32);
     return w;
 }

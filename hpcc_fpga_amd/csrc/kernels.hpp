@@ -91,9 +91,8 @@ hipError_t launch_block_inverse(const float* A, int64_t lda, int64_t n, int B, c
 // each), two flags per slice and a status word (bit 2: a bounded wait gave
 // up).  Every launch clears the flags and the status itself.
 struct GeslWs {
-    float* zf;
-    float* zb;
-    float* y;
+    float* pf;  // forward partial solutions, B per slice
+    float* pb;  // backward partial solutions, B per slice
     unsigned* flags;
     int* status;
 };

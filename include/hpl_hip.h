@@ -375,10 +375,10 @@ hpl_status hpl_kernel_bytes(hpl_ctx* ctx, double* bytes);
  * the call only enqueues work, so synchronise that stream before reading x.
  * The block is hpl_gesl's (the largest power of two <= 256 dividing n; n must
  * be a multiple of 32) and the result equals hpl_gesl's bit for bit.  The
- * diagonal-block inverses and the solve's state, about n * (396 + 8 * block)
+ * diagonal-block inverses and the solve's state, about n * (416 + 8 * block)
  * bytes, live in one fixed 40 MiB device array per GPU (held until the
  * process ends): a larger system returns HPL_ERR_BUDGET before anything is
- * enqueued (n <= 17152 when 256 divides n).  Every call on a GPU shares that array and overwrites it, so
+ * enqueued (n <= 16896 when 256 divides n).  Every call on a GPU shares that array and overwrites it, so
  * calls that overlap on one GPU (several streams or host threads) race. */
 hpl_status hpl_gesl_noalloc(const float* lu, uint64_t n, uint64_t lda, float* x, void* stream);
 

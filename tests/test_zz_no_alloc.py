@@ -96,9 +96,11 @@ def test_noalloc_refuses_sizes_and_arguments():
     hip = _Hip()
     d = hip.alloc(4096)
     try:
-        # 32768 * (384 + 8 * 256) bytes exceed the 40 MiB array: refused on
-        # the host (HPL_ERR_BUDGET), nothing enqueued
+        # n * (416 + 8 * 256) bytes exceed the 40 MiB array from n = 17152
+        # (the first multiple of 256 past 16896): refused on the host
+        # (HPL_ERR_BUDGET), nothing enqueued
         assert L.hpl_gesl_noalloc(d, 32768, 32768, d, None) == 5
+        assert L.hpl_gesl_noalloc(d, 17152, 17152, d, None) == 5
         assert L.hpl_gesl_noalloc(None, 64, 64, d, None) == 1
         assert L.hpl_gesl_noalloc(d, 64, 63, d, None) == 1   # lda < n
         assert L.hpl_gesl_noalloc(d, 64, 66, d, None) == 1   # lda not a multiple of 4
