@@ -272,7 +272,12 @@ hpl_status panel_diag(hpl_ctx* c, hipStream_t st, int64_t k, uint32_t B) {
     float* inv = c->dInv + k * (int64_t)(3 * (B / 32) * 1024);
     float* DT = c->dDT + (k & 1) * (int64_t)B * B;
     HIPCHK(c, launch_prep(c->dA, lda, off, B, DT, inv, st));
-    if (float* t = step_tinv(c, k, B)) HIPCHK(c, launch_tinv_prepare(c->dA, lda, off, B, inv, t, st));
+    // BF16: the inverses for the strip solve are the substitution's own, so
+    // they go to their slot of dBInv when the factorisation fills it
+    if (float* t = step_tinv(c, k, B))
+        HIPCHK(c, launch_tinv_prepare(c->dA, lda, off, B, inv, t,
+                                      c->binv_in_gefa ? c->dBInv + k * 2 * (int64_t)B * B : nullptr,
+                                      st));
     return HPL_SUCCESS;
 }
 
@@ -396,6 +401,14 @@ hpl_status gefa_schedule(hpl_ctx* c, uint32_t B) {
 // FP32 defaults to D = 4 with single steps in the tail (n = 32768: GEFA
 // 203.6-204.9 ms against 204.2-207.9 for D = 3; D = 4 with tail groups of 2:
 // 204.3-205.1; D = 5 and 6 with single steps: 203.8-205.6, 204.7-205.9).
+// The substitution's inverses of diagonal blocks [k0, k1) (launch_block_inverse
+// on that range of the factor and of the 32 x 32 inverses).
+hipError_t binv_range(hpl_ctx* c, hipStream_t st, int64_t k0, int64_t k1, uint32_t B) {
+    const int64_t lda = (int64_t)c->lda, is = 3 * (B / 32) * 1024;
+    return launch_block_inverse(c->dA + k0 * B * lda + k0 * B, lda, (k1 - k0) * B, (int)B,
+                                c->dInv + k0 * is, is, c->dBInv + k0 * 2 * (int64_t)B * B, st);
+}
+
 hpl_status gefa_pipelined(hpl_ctx* c, uint32_t B) {
     const int64_t n = (int64_t)c->n;
     const int64_t N = n / B;
@@ -430,6 +443,12 @@ hpl_status gefa_pipelined(hpl_ctx* c, uint32_t B) {
                 if (r) return r;
             }
         }
+        // the diagonal-block inverses of the substitution for group g-1's
+        // blocks (final; their 32 x 32 inverses come from panel_diag), off
+        // the critical path behind the bulk update; BF16 computes them per
+        // step for its strip solves
+        if (g >= 1 && c->binv_in_gefa && !step_tinv(c, bnd[g - 1], B))
+            HIPCHK(c, binv_range(c, c->s1, bnd[g - 1], bnd[g], B));
         HIPCHK(c, hipEventRecord(c->evP, c->s1));
         const int64_t k0 = g >= 1 ? bnd[g - 1] : 0;
         for (int64_t k = bnd[g]; k < bnd[g + 1]; ++k) {
@@ -485,6 +504,8 @@ hpl_status gefa_pipelined(hpl_ctx* c, uint32_t B) {
         HIPCHK(c, hipStreamWaitEvent(c->s1, c->evA, 0));
         HIPCHK(c, hipStreamWaitEvent(c->s0, c->evP, 0));
     }
+    if (G >= 1 && c->binv_in_gefa && !step_tinv(c, bnd[G - 1], B))
+        HIPCHK(c, binv_range(c, c->s0, bnd[G - 1], bnd[G], B));
     return HPL_SUCCESS;
 }
 
@@ -894,6 +915,16 @@ hpl_status hpl_gefa_dev(hpl_ctx* c, uint32_t block, hpl_prec prec, hpl_timing* t
     c->factored = false;
     c->pivoted = false;
     c->binv_ready = false;
+    // the pipelined schedule also fills the substitution's diagonal-block
+    // inverses (off its critical path), so the first solve need not
+    c->binv_in_gefa = c->lt_linear && block <= 256;
+    if (c->binv_in_gefa) {
+        const hplx::WsPlan w = hplx::plan_factor_ws(c->n, block, c->prec, 1);
+        if (ensure(c, c->dBInv, c->capBInv, w.binv) != HPL_SUCCESS) {
+            c->err.clear();
+            c->binv_in_gefa = false;  // no room: the first solve computes them
+        }
+    }
     HIPCHK(c, hipEventRecord(c->e0, c->s0));
     st = c->pivoting ? hplx::gefa_pivot(c, block)
          : c->lt_linear ? gefa_pipelined(c, block)
@@ -907,6 +938,7 @@ hpl_status hpl_gefa_dev(hpl_ctx* c, uint32_t block, hpl_prec prec, hpl_timing* t
     c->B = block;
     c->factored = true;
     c->pivoted = c->pivoting;
+    c->binv_ready = c->binv_in_gefa;
     return HPL_SUCCESS;
 }
 

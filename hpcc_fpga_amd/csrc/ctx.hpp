@@ -87,6 +87,7 @@ struct hpl_ctx {
     void* dGeslWs = nullptr;    // workspace of the single-launch substitution
     size_t capGeslWs = 0;
     bool binv_ready = false;    // dBInv holds the inverses of the current factor
+    bool binv_in_gefa = false;  // the running factorisation fills dBInv
     float* dInner = nullptr;    // recursion scratch for blocks > 128
     size_t capInner = 0;
     // BF16 factorisation: the current steps' diagonal-block inverses for

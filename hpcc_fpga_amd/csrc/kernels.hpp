@@ -24,15 +24,17 @@ hipError_t launch_trsm(float* A, int64_t lda, int64_t off, int64_t m, int B, con
 // Strip solves of the BF16 factorisation through the diagonal block's full
 // inverses (B <= 256): launch_tinv_prepare turns the block at (off, off) --
 // factorised, with its 32 x 32 inverses in inv (launch_prep) -- into both
-// inverses in `tinv` (tinv_floats(B) floats: OutL then OutU as
-// launch_block_inverse writes them, followed by their MFMA operand copies);
+// inverses (OutL then OutU as launch_block_inverse writes them: at `out`, the
+// block's slot of the substitution's inverses, or, when out is null, at the
+// start of `tinv`) and their MFMA operand copies in `tinv` (tinv_floats(B)
+// floats, the copies after the first 2 B^2);
 // launch_trsm_inv then solves both strips as one product each, y = G v, on
 // BF16 MFMA with the operands split into high and low BF16 halves (three
 // products: about 2^-17 relative), writing the FP32 strips and their
 // k-contiguous 16-bit copies like launch_trsm (lp_prec 1: BF16; 2: FP16).
 int64_t tinv_floats(int B);
 hipError_t launch_tinv_prepare(const float* A, int64_t lda, int64_t off, int B, const float* inv,
-                               float* tinv, hipStream_t s);
+                               float* tinv, float* out, hipStream_t s);
 hipError_t launch_trsm_inv(float* A, int64_t lda, int64_t off, int64_t m, int B,
                            const float* tinv, hipStream_t s, int lp_prec, void* lpT, void* lpL,
                            int64_t ldlp);

@@ -761,18 +761,19 @@ int64_t tinv_floats(int B) {
 }
 
 hipError_t launch_tinv_prepare(const float* A, int64_t lda, int64_t off, int B, const float* inv,
-                               float* tinv, hipStream_t s) {
+                               float* tinv, float* out, hipStream_t s) {
     if (B % 32 || B > 256 || B < 32 || (reinterpret_cast<uintptr_t>(tinv) & 15))
         return hipErrorInvalidValue;
     const int NQ = B / 32;
     const float* D = A + off * lda + off;
     bf16x8_t* pk = reinterpret_cast<bf16x8_t*>(tinv + 2 * (int64_t)B * B);
+    float* o = out ? out : tinv;
     dim3 g1(NQ, 2, 1), g2(NQ * (NQ + 1) / 2, 2);
     switch (B) {
 #define HPL_TINV(BB)                                                                     \
     case BB:                                                                             \
-        block_inverse_kernel<BB><<<g1, 64, 0, s>>>(D, lda, inv, 0, tinv);                \
-        tinv_pack_kernel<BB><<<g2, 64, 0, s>>>(tinv, pk);                                \
+        block_inverse_kernel<BB><<<g1, 64, 0, s>>>(D, lda, inv, 0, o);                   \
+        tinv_pack_kernel<BB><<<g2, 64, 0, s>>>(o, pk);                                   \
         break;
         HPL_TINV(32)
         HPL_TINV(64)
