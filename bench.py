@@ -533,6 +533,9 @@ def main():
     ap.add_argument("--fp32-steps", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32", action="store_true", help="skip the plain FP32 section")
+    ap.add_argument("--only-fp32", action="store_true",
+                    help="run the plain FP32 section alone and print it as the JSON line "
+                         "(for profiling that section)")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the pivoted, FP64, mixed-precision sweep, batched and serving "
                          "sections")
@@ -554,6 +557,11 @@ def main():
     ctx = Context(0)
     dev = ctx.device_name
     print(f"[bench] device {dev}", file=sys.stderr, flush=True)
+    if args.only_fp32:
+        sec = fp32_section(ctx, args.fp32_n, B, args.fp32_steps, args.warmup)
+        ctx.close()
+        print(json.dumps({"fp32_plain": sec, "device": dev}), flush=True)
+        return
     h = headline(ctx, n, B, args.d, args.steps, args.warmup)
     tot = h["tot"]
 

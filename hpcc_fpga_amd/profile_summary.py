@@ -1,7 +1,8 @@
 """Summaries of rocprofv3 output for the profiles/ folder.
 
     python -m hpcc_fpga_amd.profile_summary traffic --fetch DIR --write DIR --bench JSON
-        --kernels PAT[,PAT] [--min-grid G] --copy-bytes N --out profiles/X.json --md profiles/X.md
+        [--bench-key SECTION] --kernels PAT[,PAT] [--min-grid G] --copy-bytes N
+        --out profiles/X.json --md profiles/X.md
     python -m hpcc_fpga_amd.profile_summary mfma --pmc DIR --kernels PAT[,PAT] [--min-grid G] --md M
     python -m hpcc_fpga_amd.profile_summary trace --trace CSV --md M [--kernels PAT --flops F]
     python -m hpcc_fpga_amd.profile_summary bandwidth --fetch DIR --write DIR --copy-bytes N
@@ -76,7 +77,7 @@ def cmd_traffic(a):
     with open(a.bench) as fh:
         line = [ln for ln in fh if ln.strip().startswith("{")][-1]
     bench = json.loads(line)
-    rf = bench["roofline"]
+    rf = bench[a.bench_key]["roofline"] if a.bench_key else bench["roofline"]
     alg = rf["algorithmic_bytes_per_launch"]
     res = {"kernels": pats, "launches_counted": n, "launches_expected": rf["launches"],
            "fetch_scale": fs, "write_scale": ws, "copy_fetch_kib": fkib, "copy_write_kib": wkib,
@@ -207,6 +208,8 @@ def main(argv=None):
     sub.choices["traffic"].add_argument("--fetch", required=True)
     sub.choices["traffic"].add_argument("--write", required=True)
     sub.choices["traffic"].add_argument("--bench", required=True)
+    sub.choices["traffic"].add_argument("--bench-key", default="",
+                                        help="section of the bench JSON holding the roofline")
     sub.choices["traffic"].add_argument("--copy-bytes", type=float, required=True)
     sub.choices["traffic"].add_argument("--out", required=True)
     sub.choices["mfma"].add_argument("--pmc", required=True)
