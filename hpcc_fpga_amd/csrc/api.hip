@@ -410,12 +410,13 @@ hipError_t binv_range(hpl_ctx* c, hipStream_t st, int64_t k0, int64_t k1, uint32
 }
 
 hpl_status gefa_pipelined(hpl_ctx* c, uint32_t B) {
+    hpl_status r = hplx::init_pipeline_streams(c);
+    if (r) return r;
     const int64_t n = (int64_t)c->n;
     const int64_t N = n / B;
     const std::vector<int64_t> bnd =
         hplx::group_bounds(c, N, B, c->depth, c->prec == HPL_PREC_FP32 ? 1 : 16);
     const int64_t G = (int64_t)bnd.size() - 1;
-    hpl_status r;
     for (int64_t g = 0; g < G; ++g) {
         const int64_t s = bnd[g + 1] * (int64_t)B;
         const bool wide = (c->prec == HPL_PREC_BF16 || c->prec == HPL_PREC_FP16) &&
@@ -641,11 +642,21 @@ IrPlan plan_ir(uint64_t n, int nrhs) {
 hpl_status init_streams(hpl_ctx* c) {
     if (hipStreamCreateWithFlags(&c->s0, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&c->s1, hipStreamNonBlocking) != hipSuccess ||
-        hipStreamCreateWithFlags(&c->s2, hipStreamNonBlocking) != hipSuccess ||
-        hipStreamCreateWithFlags(&c->s3, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->e0) != hipSuccess || hipEventCreate(&c->e1) != hipSuccess ||
         hipEventCreateWithFlags(&c->evA, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c->evP, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->evP, hipEventDisableTiming) != hipSuccess)
+        return fail(c, HPL_ERR_HIP, "stream or event creation failed");
+    return HPL_SUCCESS;
+}
+
+// The pipelined schedule's two further streams and their events, made on its
+// first use: contexts that never run it (the serving engine's slots) keep
+// two streams each, so several of them do not crowd the device's few hardware
+// queues.
+hpl_status init_pipeline_streams(hpl_ctx* c) {
+    if (c->s2) return HPL_SUCCESS;
+    if (hipStreamCreateWithFlags(&c->s2, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->s3, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&c->evS, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&c->evD, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&c->evL, hipEventDisableTiming) != hipSuccess)

@@ -183,6 +183,7 @@ hpl_status fail(hpl_ctx* c, hpl_status st, const char* fmt, ...);
 
 // Streams and events of a context (default priority, non-blocking).
 hpl_status init_streams(hpl_ctx* c);
+hpl_status init_pipeline_streams(hpl_ctx* c);
 
 // Device bytes of the factorisation workspaces of an n x n system at block B
 // with the given trailing-update precision and update depth.
