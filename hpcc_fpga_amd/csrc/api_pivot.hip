@@ -37,12 +37,12 @@ using hplx::fail;
 
 // Byte offsets inside the context's pivot block.
 struct PvLayout {
-    size_t piv, perm, src, diag, pub, krow, keys, sync, list, y, d, total;
+    size_t piv, perm, src, diag, xw, list, y, d, total;
 };
 
 size_t al256(size_t x) { return (x + 255) / 256 * 256; }
 
-PvLayout pv_layout(uint64_t n, uint32_t B, int64_t groups) {
+PvLayout pv_layout(uint64_t n, uint32_t B) {
     PvLayout L{};
     size_t o = 0;
     auto take = [&](size_t bytes) {
@@ -54,10 +54,7 @@ PvLayout pv_layout(uint64_t n, uint32_t B, int64_t groups) {
     L.perm = take(n * sizeof(int32_t));
     L.src = take(n * sizeof(int32_t));
     L.diag = take(n * sizeof(float));
-    L.pub = take(2 * (size_t)groups * (B + 1) * sizeof(float));
-    L.krow = take(2 * (size_t)(B + 1) * sizeof(float));
-    L.keys = take((size_t)B * sizeof(unsigned long long));
-    L.sync = take(sizeof(unsigned));
+    L.xw = take(panel_exchange_words((int64_t)n, (int)B) * sizeof(unsigned long long));
     L.list = take((4 * (size_t)B + 1) * sizeof(int));
     L.y = take(2 * (size_t)B * n * sizeof(float));
     L.d = take((size_t)B * B * sizeof(float));
@@ -68,9 +65,7 @@ PvLayout pv_layout(uint64_t n, uint32_t B, int64_t groups) {
 // Allocate the pivot block for order c->n at block B and point dPiv, dPerm
 // and dDiagU into it.
 hpl_status bind_pivot_block(hpl_ctx* c, uint32_t B, PvLayout* out) {
-    const int R = panel_pivot_rows((int)B);
-    const int64_t groups = ((int64_t)c->n + R - 1) / R;
-    const PvLayout L = pv_layout(c->n, B, groups);
+    const PvLayout L = pv_layout(c->n, B);
     hpl_status st = hplx::ensure_mem(c, &c->dPv, &c->capPv, L.total);
     if (st) return st;
     char* base = static_cast<char*>(c->dPv);
@@ -118,10 +113,11 @@ hpl_status gefa_pivot(hpl_ctx* c, uint32_t B) {
     if (st) return st;
     char* base = static_cast<char*>(c->dPv);
     int* src = reinterpret_cast<int*>(base + L.src);
-    float* pub = reinterpret_cast<float*>(base + L.pub);
-    float* krow = reinterpret_cast<float*>(base + L.krow);
-    auto* keys = reinterpret_cast<unsigned long long*>(base + L.keys);
-    auto* sync = reinterpret_cast<unsigned*>(base + L.sync);
+    // the panel exchange: candidate rows, row k, key slots (groups + 1 per
+    // parity, the extra one for the error-path test's phantom workgroup)
+    auto* pub = reinterpret_cast<unsigned long long*>(base + L.xw);
+    auto* krow = pub + 2 * groups * (B + 1);
+    auto* kslot = krow + 2 * (B + 1);
     int* list = reinterpret_cast<int*>(base + L.list);
     int* count = list + 4 * B;
     float* Y = reinterpret_cast<float*>(base + L.y);
@@ -129,13 +125,14 @@ hpl_status gefa_pivot(hpl_ctx* c, uint32_t B) {
     hipStream_t s = c->s0;
     const int64_t N = n / B;
     const int64_t inv_stride = 3 * (B / 32) * 1024;
+    PCHK(c, launch_zero_words(pub, panel_exchange_words(n, (int)B), s));
     PCHK(c, launch_transpose_square(c->dA, lda, n, s));
     PCHK(c, launch_iota(c->dPerm, n, s));
     for (int64_t k = 0; k < N; ++k) {
         const int64_t k0 = k * B, m1 = n - k0 - B;
-        PCHK(c, launch_panel_pivot(c->dA, lda, n, k0, (int)B, pub, krow, keys, sync, c->dLt, n,
-                                   c->dDiagU, c->dPiv, src, c->dStatus, c->pv_max_groups, s,
-                                   c->pv_spin_limit));
+        PCHK(c, launch_panel_pivot(c->dA, lda, n, k0, (int)B, pub, krow, kslot, (int)groups + 1,
+                                   c->dLt, n, c->dDiagU, c->dPiv, src, c->dStatus,
+                                   c->pv_max_groups, c->num_cu, s, c->pv_spin_limit));
         PCHK(c, launch_row_moves(c->dA, lda, n, k0, (int)B, c->dPiv, src, list, count, Y, n,
                                  c->dPerm, c->dStatus, s));
         if (m1 > 0) {

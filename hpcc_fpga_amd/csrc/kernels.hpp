@@ -118,26 +118,33 @@ hipError_t launch_gesl_multi(const float* A, int64_t lda, int64_t n, int B, cons
 
 // partial pivoting (pivot.hip); X is the transposed matrix X[i][j] = M(i, j)
 hipError_t launch_transpose_square(float* A, int64_t ld, int64_t n, hipStream_t s);
-// rows per workgroup and dynamic LDS bytes of the panel kernel at block B;
-// workgroups of a panel of m rows
+// rows per workgroup of the panel kernel at block B; workgroups of a panel
+// of m rows
 int panel_pivot_rows(int B);
-size_t panel_pivot_lds(int B);
 int panel_pivot_groups(int64_t m, int B);
-// allow the panel's LDS and return how many of its workgroups are co-resident
+// set the panel kernel up and return how many of its workgroups are co-resident
 hipError_t panel_pivot_prepare(int B, int* max_groups);
 // factorise panel k0 (m = n - k0 rows of X, B columns) with partial pivoting;
-// pub: 2 * G * (B+1) floats, krow: 2 * (B+1), keys: B, ctr: one counter
-// (G = panel_pivot_groups(m, B) <= max_groups); writes the panel (-1/u on
-// the diagonal), Lt[c][pos] for pos >= k0 + B, diagU/ipvt at k0.., and the row
-// of origin src[pos] of positions k0..n-1.  status |= 1: zero or non-finite
-// pivot; |= 2: the grid barrier timed out; |= 4: a pivot key outside the
-// panel was read (a broken barrier; the row is left in place).  spin_limit:
-// polls of the barrier before a workgroup gives up (0: about a second; < 0:
-// give up after -spin_limit polls on every barrier, the error path).
-hipError_t launch_panel_pivot(float* X, int64_t ldx, int64_t n, int64_t k0, int B, float* pub,
-                              float* krow, unsigned long long* keys, unsigned* ctr, float* Lt,
-                              int64_t ldl, float* diagU, int* ipvt, int* src, int* status,
-                              int max_groups, hipStream_t s, long spin_limit = 0);
+// pub, krow, kslot: the exchange words of the panel's workgroups (G =
+// panel_pivot_groups(m, B) <= max_groups; kslot has gslots > G slots per
+// parity), carved from panel_exchange_words(n, B) words that
+// launch_zero_words clears before each factorisation; writes the panel (-1/u
+// on the diagonal), Lt[c][pos] for pos >= k0 + B, diagU/ipvt at k0.., and the
+// row of origin src[pos] of positions k0..n-1.  status |= 1: zero or
+// non-finite pivot; |= 2: an exchange timed out; |= 4: a pivot key outside
+// the panel was read (a broken exchange; the row is left in place).
+// spin_limit: polls of a wait before a workgroup gives up (0: about a second;
+// < 0: give up after -spin_limit polls on every exchange, the error path).
+// A grid of at most num_cu workgroups gets one CU per workgroup.
+hipError_t launch_panel_pivot(float* X, int64_t ldx, int64_t n, int64_t k0, int B,
+                              unsigned long long* pub, unsigned long long* krow,
+                              unsigned long long* kslot, int gslots, float* Lt, int64_t ldl,
+                              float* diagU, int* ipvt, int* src, int* status, int max_groups,
+                              int num_cu, hipStream_t s, long spin_limit = 0);
+// 64-bit words of the panel exchange for order n at block B: pub (2 G (B+1)),
+// krow (2 (B+1)), kslot (2 (G+1) slots of 8), G = panel_pivot_groups(n, B)
+size_t panel_exchange_words(int64_t n, int B);
+hipError_t launch_zero_words(unsigned long long* p, size_t words, hipStream_t s);
 // apply panel k0's row moves to the columns outside it and to perm; list:
 // 4 B ints, count: one int, Y: 2 B x ldy floats.  Nothing moves when status
 // carries a panel failure (bits 2, 4) or a listed position lies outside

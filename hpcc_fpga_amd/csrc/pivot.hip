@@ -13,26 +13,39 @@
 // Panel.  One persistent launch factorises the m x B panel (m = n - k0):
 // workgroup g keeps rows [k0 + gR, k0 + gR + R) of the panel in LDS for all B
 // columns.  Column c needs the whole column's maximum, so every column ends in
-// one grid-wide barrier:
+// one grid-wide exchange:
 //   * each workgroup publishes its best candidate as a 64-bit key
 //     (|value| bits << 32 | ~position: max = largest magnitude, then smallest
-//     position, the first-index rule of gefa_ref's strict '<', :80-86) with
-//     one atomicMax, and writes its candidate row and (owner of row k) row k
-//     to double-buffered slots;
-//   * after the barrier every workgroup reads the winning key, copies the
-//     winning row, exchanges rows k and p where it owns them, and applies
-//     gefa_ref's arithmetic to its rows: multiplier l = M(i,k) * (-1.0/u) in
-//     double rounded to float (:96-98), update M(i,j) += l * u_j as a float
-//     product and a float sum (:100-105).
+//     position, the first-index rule of gefa_ref's strict '<', :80-86), the
+//     candidate's values at columns c and c+1, its whole candidate row and
+//     (owner of row k) row k and its values at c and c+1;
+//   * every workgroup polls all G key slots; once each holds column c's key,
+//     it knows the winner and the values at c and c+1 of the winning row and
+//     of row k, forms the multipliers of column c and column c+1 and searches
+//     column c+1 while the two whole rows it needs are still in flight;
+//   * it applies gefa_ref's arithmetic to its rows: multiplier l = M(i,k) *
+//     (-1.0/u) in double rounded to float (:96-98), update M(i,j) += l * u_j as
+//     a float product and a float sum (:100-105).
+// Every word exchanged is 64 bits, (tag << 32) | payload, with tag = the
+// global column + 1, written and read with single-copy-atomic agent-scope
+// accesses.  A reader accepts a word only when it carries the column's tag,
+// so a writer never waits for its stores to complete and no counter, fence
+// or atomic read-modify-write is needed: one store-to-load trip per column
+// plus the fetch of the winning row.  Slots alternate by column parity: a
+// slot of column c is rewritten at column c+2, after every workgroup has
+// arrived at column c+1, which each does only after reading column c's
+// words.  The host zeroes the slots before each factorisation, so tags are
+// unique since the last clear.
 // Whole rows move (all B columns), so inside the panel the multipliers follow
 // LAPACK's convention (every later exchange applied); the exchanges of the
 // other columns follow as one gather of the moved rows.  A factor with only
 // one panel is bit-identical to gefa_ref after the exchanges of the earlier
 // columns are undone (a data movement done on download).
 //
-// The barrier spins with a bound: a workgroup that waits longer than about a
-// second raises the status and releases every waiter, so every wave exits
-// (the host sizes the grid to what is co-resident, so this is an error path).
+// Every wait spins with a bound: a workgroup that waits longer than about a
+// second raises the status, marks its key slots with the abort tag (which
+// every waiter takes as a release) and exits, so every wave exits (the host
+// sizes the grid to what is co-resident, so this is an error path).
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -40,9 +53,11 @@
 
 namespace {
 
-constexpr int PNT = 256;                // threads of the panel kernel
-constexpr unsigned kAbort = 1u << 30;   // added to the barrier counter on abort
-constexpr long kSpinLimit = 1L << 21;   // default polls (each >= ~0.5 us) before abort
+constexpr int PNT = 512;                   // threads of the panel kernel
+constexpr int kMaxPanelGroups = 4 * PNT;   // key slots one workgroup polls
+constexpr int kSlotWords = 8;              // words of a key slot (6 used)
+constexpr unsigned kAbortTag = 0xFFFFFFFFu;
+constexpr long kSpinLimit = 1L << 21;      // default polls (each >= ~0.5 us) before abort
 
 // ---------------------------------------------------------------- transpose
 // In-place transpose of an n x n matrix: workgroup (J, I), I <= J, exchanges
@@ -94,46 +109,50 @@ __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
     return v;
 }
 
+// 64-bit exchange words: (tag << 32) | payload
+__device__ __forceinline__ void put_tagged(unsigned long long* p, unsigned payload, unsigned tag) {
+    __hip_atomic_store(p, ((unsigned long long)tag << 32) | payload, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned long long get_tagged(const unsigned long long* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned word_tag(unsigned long long w) { return (unsigned)(w >> 32); }
+__device__ __forceinline__ unsigned word_payload(unsigned long long w) { return (unsigned)w; }
+
 // LDS: tile[R][B + 1] (column B holds the row's position of origin as int
 // bits) | urow[B + 1] (the pivot row of the current column) | lcol[R] (the
 // multipliers of the current column).
 //
-// Barrier of column c: thread 0 of every workgroup folds its key into the
-// column's key (atomicMax), waits for that to complete, and counts the
-// workgroup in on one counter; every workgroup polls the counter until all G
-// have arrived (an abort adds kAbort).  What workgroups exchange -- the
-// candidate rows and row k -- is written and read with agent-scope coherent
-// accesses, and every wave drains its stores before the arrival, so no L2
-// writeback or invalidate is needed (those serialise across the device: with
-// them a barrier of 256 workgroups measured 18 us, without 6 us).
-__device__ __forceinline__ void publish(float* p, float v) {
-    __hip_atomic_store(reinterpret_cast<unsigned*>(p), __float_as_uint(v), __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ float fetch(const float* p) {
-    return __uint_as_float(__hip_atomic_load(reinterpret_cast<const unsigned*>(p), __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_AGENT));
-}
-
+// Exchange buffers (64-bit words, see above): pub[2][G][B+1] candidate rows,
+// krow[2][B+1] row k, kslot[2][gslots][kSlotWords] key slots: words 0/1 the
+// key's low/high half, 2/3 the candidate's values at columns c and c+1, 4/5
+// row k's (written by its owner only).
 template <int B>
 __global__ __launch_bounds__(PNT) void panel_pivot_kernel(
-    float* __restrict__ X, int64_t ldx, int64_t n, int64_t k0, int R, float* __restrict__ pub,
-    float* __restrict__ krow, unsigned long long* __restrict__ keys, unsigned* __restrict__ ctr,
-    float* __restrict__ Lt, int64_t ldl, float* __restrict__ diagU, int* __restrict__ ipvt,
-    int* __restrict__ src, int* __restrict__ status, long spin_limit, unsigned phantom) {
+    float* __restrict__ X, int64_t ldx, int64_t n, int64_t k0, int R,
+    unsigned long long* __restrict__ pub, unsigned long long* __restrict__ krow,
+    unsigned long long* __restrict__ kslot, int gslots, float* __restrict__ Lt, int64_t ldl,
+    float* __restrict__ diagU, int* __restrict__ ipvt, int* __restrict__ src,
+    int* __restrict__ status, long spin_limit, unsigned phantom) {
 #pragma clang fp contract(off)
     constexpr int LDT = B + 1;
     constexpr int TPC = PNT / B;  // threads per column in the update
+    constexpr int NF = (B + PNT) / PNT;  // row words per thread (B + 1 of them)
     extern __shared__ __attribute__((aligned(16))) float sm[];
     float* tile = sm;
     float* urow = sm + (size_t)R * LDT;
     float* lcol = urow + LDT;
-    __shared__ unsigned long long wkey[PNT / 64];
+    __shared__ unsigned long long skey[PNT / 64], wkey[PNT / 64];
+    __shared__ float wval[PNT / 64][2];
+    __shared__ float kval[2];
     __shared__ int abort_flag;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int G = gridDim.x, g = blockIdx.x;
     const int64_t p0 = k0 + (int64_t)g * R;  // position of local row 0
     const int rows = (int)min<int64_t>((int64_t)R, n - p0);
+    const size_t kstride = (size_t)gslots * kSlotWords;
+    if (t == 0) abort_flag = 0;
 
     // the workgroup's rows of the panel as float4s, 16 per thread in flight at
     // a time (ldx and k0 are multiples of 4, so the rows are 16-byte aligned)
@@ -165,109 +184,208 @@ __global__ __launch_bounds__(PNT) void panel_pivot_kernel(
     for (int r = t; r < rows; r += PNT) tile[r * LDT + B] = __int_as_float((int)(p0 + r));
     __syncthreads();
 
-    // Column c's candidate search, the publication of the candidate row and
-    // of row k, and the arrival at column c's barrier.  With upd (every
-    // column after the first) the workgroup has applied the previous column's
-    // rank-1 update to column c only: the published rows get it on the fly
-    // for their columns beyond c (the same float product and sum), and the
-    // workgroup updates those columns after arriving, while the other
-    // workgroups arrive (a split barrier).
-    unsigned long long best = 0;
-    auto search_arrive = [&](int c, bool upd) {
+    // the workgroup's best candidate of column c (rows at positions >= k)
+    auto search = [&](int c) {
         const int64_t k = k0 + c;
+        unsigned long long b = 0;
+        for (int r = (int)max<int64_t>(0, k - p0) + t; r < rows; r += PNT)
+            b = max(b, pivot_key(tile[r * LDT + c], p0 + r, k));
+        b = wave_max_u64(b);
+        if (lane == 0) skey[wave] = b;
+        __syncthreads();
+        b = skey[0];
+#pragma unroll
+        for (int w = 1; w < PNT / 64; ++w) b = max(b, skey[w]);
+        return b;
+    };
+    // Publish column c's candidate `best`: the whole row, row k when this
+    // workgroup owns it, then the key slot.  With upd (every column after the
+    // first) the workgroup has applied the previous column's rank-1 update to
+    // column c only: the published rows get it on the fly for their columns
+    // beyond c (the same float product and sum), and the workgroup updates
+    // those columns after publishing, while the others arrive.
+    auto publish = [&](int c, unsigned long long best, bool upd) {
+        const int64_t k = k0 + c;
+        const unsigned tag = (unsigned)(k + 1);
         const int buf = c & 1;
         const int64_t rk = k - p0;
-        best = 0;
-        for (int r = (int)max<int64_t>(0, rk) + t; r < rows; r += PNT)
-            best = max(best, pivot_key(tile[r * LDT + c], p0 + r, k));
-        best = wave_max_u64(best);
-        if (lane == 0) wkey[wave] = best;
-        __syncthreads();
-        best = wkey[0];
-#pragma unroll
-        for (int w = 1; w < PNT / 64; ++w) best = max(best, wkey[w]);
+        const bool own_k = rk >= 0 && rk < rows;
         auto cur = [&](int r, int j) {
             float v = tile[r * LDT + j];
             if (upd && j > c && j < B) v = v + lcol[r] * urow[j];
             return v;
         };
+        const int rl = best != 0 ? (int)(key_position(best, k) - p0) : 0;
         if (best != 0) {
-            float* slot = pub + ((int64_t)buf * G + g) * LDT;
-            const int rl = (int)(key_position(best, k) - p0);
-            for (int j = t; j <= B; j += PNT) publish(slot + j, cur(rl, j));
+            unsigned long long* slot = pub + ((int64_t)buf * G + g) * LDT;
+            for (int j = t; j <= B; j += PNT) put_tagged(slot + j, __float_as_uint(cur(rl, j)), tag);
         }
-        if (rk >= 0 && rk < rows)
-            for (int j = t; j <= B; j += PNT) publish(krow + buf * LDT + j, cur((int)rk, j));
-        __builtin_amdgcn_s_waitcnt(0);  // this wave's stores have completed
-        __syncthreads();
+        if (own_k)
+            for (int j = t; j <= B; j += PNT)
+                put_tagged(krow + buf * LDT + j, __float_as_uint(cur((int)rk, j)), tag);
         if (t == 0) {
+            unsigned long long* w = kslot + (size_t)buf * kstride + (size_t)g * kSlotWords;
             if (best != 0) {
-                atomicMax(keys + c, best);
-                __builtin_amdgcn_s_waitcnt(0);
+                put_tagged(w + 2, __float_as_uint(cur(rl, c)), tag);
+                put_tagged(w + 3, c + 1 < B ? __float_as_uint(cur(rl, c + 1)) : 0u, tag);
             }
-            __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (own_k) {
+                put_tagged(w + 4, __float_as_uint(cur((int)rk, c)), tag);
+                put_tagged(w + 5, c + 1 < B ? __float_as_uint(cur((int)rk, c + 1)) : 0u, tag);
+            }
+            put_tagged(w + 0, (unsigned)best, tag);
+            put_tagged(w + 1, (unsigned)(best >> 32), tag);
         }
     };
-    // wait until all G workgroups have arrived at column c's barrier; true
-    // when the panel was aborted (a wait past the spin limit)
-    auto wait_all = [&](int c) {
+    // give up: raise the status, mark this workgroup's key slots (both
+    // parities) with the abort tag so that every waiter is released
+    auto give_up = [&]() {
         if (t == 0) {
-            // phantom > 0 (a test setting) waits for arrivals that never come
-            const unsigned target = (unsigned)(c + 1) * (unsigned)G + phantom;
-            unsigned v;
+            atomicOr(status, 2);
+            for (int b2 = 0; b2 < 2; ++b2) {
+                unsigned long long* w = kslot + (size_t)b2 * kstride + (size_t)g * kSlotWords;
+                put_tagged(w + 0, 0u, kAbortTag);
+                put_tagged(w + 1, 0u, kAbortTag);
+            }
+        }
+    };
+    // Column c's exchange: poll the key slots of all G groups (and the
+    // phantom ones of the error-path test, which nobody writes) until each
+    // carries column c's tag.  Returns the winning key, the winner's values at
+    // c and c+1 in *u0, *u1; row k's land in kval.  False in *ok when the wait
+    // gave up or another workgroup did.  wkey / wval are next written after
+    // several barriers, so they need no barrier after the reads here.
+    auto exchange = [&](int c, bool* ok, float* u0, float* u1) {
+        const unsigned tag = (unsigned)(k0 + c + 1);
+        const unsigned long long* ks = kslot + (size_t)(c & 1) * kstride;
+        const int GW = G + (int)phantom;
+        const int gk = c / R;  // the group holding position k
+        unsigned long long mx = 0;
+        float m0 = 0.0f, m1 = 0.0f;
+        bool fail = false;
+        for (int q = t; q < GW && !fail; q += PNT) {
+            const unsigned long long* w = ks + (size_t)q * kSlotWords;
+            const bool want_k = q == gk;
+            unsigned long long v[6];
             long spins = 0;
-            while ((v = __hip_atomic_load(ctr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) <
-                   target) {
-                __builtin_amdgcn_s_sleep(1);
-                if (++spins > spin_limit) {
-                    atomicOr(status, 2);
-                    v = __hip_atomic_fetch_add(ctr, kAbort, __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_AGENT) +
-                        kAbort;
+            for (;;) {
+#pragma unroll
+                for (int i = 0; i < 6; ++i) v[i] = get_tagged(w + i);
+                bool have = word_tag(v[0]) == tag && word_tag(v[1]) == tag;
+                if (have) {
+                    const unsigned long long key =
+                        ((unsigned long long)word_payload(v[1]) << 32) | word_payload(v[0]);
+                    if (key != 0) have = word_tag(v[2]) == tag && word_tag(v[3]) == tag;
+                }
+                if (want_k) have = have && word_tag(v[4]) == tag && word_tag(v[5]) == tag;
+                if (have) break;
+                if (word_tag(v[0]) == kAbortTag || word_tag(v[1]) == kAbortTag ||
+                    ++spins > spin_limit) {
+                    fail = true;
                     break;
                 }
+                __builtin_amdgcn_s_sleep(1);
             }
-            abort_flag = v >= kAbort;
+            if (fail) break;
+            const unsigned long long key =
+                ((unsigned long long)word_payload(v[1]) << 32) | word_payload(v[0]);
+            if (key > mx) {
+                mx = key;
+                m0 = __uint_as_float(word_payload(v[2]));
+                m1 = __uint_as_float(word_payload(v[3]));
+            }
+            if (want_k) {
+                kval[0] = __uint_as_float(word_payload(v[4]));
+                kval[1] = __uint_as_float(word_payload(v[5]));
+            }
+        }
+        if (fail) abort_flag = 1;
+        // the maximum key with its values, over the wave, then the workgroup
+        for (int o = 32; o > 0; o >>= 1) {
+            const unsigned lo = __shfl_xor((unsigned)mx, o);
+            const unsigned hi = __shfl_xor((unsigned)(mx >> 32), o);
+            const float o0 = __shfl_xor(m0, o), o1 = __shfl_xor(m1, o);
+            const unsigned long long om = ((unsigned long long)hi << 32) | lo;
+            if (om > mx) {
+                mx = om;
+                m0 = o0;
+                m1 = o1;
+            }
+        }
+        if (lane == 0) {
+            wkey[wave] = mx;
+            wval[wave][0] = m0;
+            wval[wave][1] = m1;
         }
         __syncthreads();
-        return abort_flag != 0;
+        *ok = abort_flag == 0;
+        int bw = 0;
+        mx = wkey[0];
+#pragma unroll
+        for (int w = 1; w < PNT / 64; ++w)
+            if (wkey[w] > mx) {
+                mx = wkey[w];
+                bw = w;
+            }
+        *u0 = wval[bw][0];
+        *u1 = wval[bw][1];
+        return mx;
     };
 
-    search_arrive(0, false);
+    publish(0, search(0), false);
     for (int c = 0; c < B; ++c) {
-        if (wait_all(c)) return;
+        bool ok = true;
+        float piv = 0.0f, u1 = 0.0f;
+        const unsigned long long win = exchange(c, &ok, &piv, &u1);
+        if (!ok) {
+            give_up();
+            return;
+        }
         const int64_t k = k0 + c;
+        const unsigned tag = (unsigned)(k + 1);
         const int buf = c & 1;
         const int64_t rk = k - p0;  // local row of position k (may be outside)
-        // ---- pivot, exchange
-        const unsigned long long kk =
-            __hip_atomic_load(keys + c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        int64_t p = key_position(kk, k);
-        if (p < k || p >= n) {  // cannot happen with a complete barrier; never index with it
+        int64_t p = key_position(win, k);
+        if (p < k || p >= n) {  // cannot happen with a complete exchange; never index with it
             if (t == 0) atomicOr(status, 4);
             p = k;
         }
         const int gw = (int)((p - k0) / R);
-        const float* wrow = pub + ((int64_t)buf * G + gw) * LDT;
-        for (int j = t; j <= B; j += PNT) urow[j] = fetch(wrow + j);
-        __syncthreads();
-        if (p != k) {
-            if (rk >= 0 && rk < rows)
-                for (int j = t; j <= B; j += PNT) tile[rk * LDT + j] = urow[j];
-            const int64_t rp = p - p0;
-            if (rp >= 0 && rp < rows)
-                for (int j = t; j <= B; j += PNT) tile[rp * LDT + j] = fetch(krow + buf * LDT + j);
+        const int64_t rp = p - p0;
+        const bool swap = p != k;
+        const bool own_k = swap && rk >= 0 && rk < rows;
+        const bool own_p = swap && rp >= 0 && rp < rows;
+        // the whole winning row (and row k where position p is ours) start
+        // now and are checked after the search of column c+1
+        const unsigned long long* wrow = pub + ((int64_t)buf * G + gw) * LDT;
+        const unsigned long long* krw = krow + buf * LDT;
+        unsigned long long fw[NF], fk[NF];
+#pragma unroll
+        for (int i = 0; i < NF; ++i) {
+            const int j = t + i * PNT;
+            fw[i] = j <= B ? get_tagged(wrow + j) : 0ull;
+            fk[i] = (own_p && j <= B) ? get_tagged(krw + j) : 0ull;
         }
-        const float piv = urow[c];
-        if (g == 0 && t == 0) {
-            ipvt[k] = (int)p;
-            diagU[k] = piv;
-            if (!(piv != 0.0f) || !__builtin_isfinite(piv)) atomicOr(status, 1);
+        if (t == 0) {
+            if (g == 0) {
+                ipvt[k] = (int)p;
+                diagU[k] = piv;
+                if (!(piv != 0.0f) || !__builtin_isfinite(piv)) atomicOr(status, 1);
+            }
+            // the exchanged rows' values at columns c and c+1 (the rest of
+            // the rows follows from the fetches)
+            if (own_k) {
+                tile[rk * LDT + c] = piv;
+                if (c + 1 < B) tile[rk * LDT + c + 1] = u1;
+            }
+            if (own_p) {
+                tile[rp * LDT + c] = kval[0];
+                if (c + 1 < B) tile[rp * LDT + c + 1] = kval[1];
+            }
         }
         __syncthreads();
         // ---- multipliers of the rows below position k (gefa_ref: double
-        // reciprocal and product, rounded once), then the rank-1 update as a
-        // float product and a float sum
+        // reciprocal and product, rounded once)
         const int rs1 = (int)max<int64_t>(0, k + 1 - p0);
         const double inv = -1.0 / (double)piv;
         for (int r = rs1 + t; r < rows; r += PNT) {
@@ -276,29 +394,63 @@ __global__ __launch_bounds__(PNT) void panel_pivot_kernel(
             lcol[r] = l;
         }
         __syncthreads();
-        if (c + 1 == B) break;
-        // column c+1 first, one row per thread: the next search reads it
-        {
-            const float u1 = urow[c + 1];
+        unsigned long long next = 0;
+        if (c + 1 < B) {
+            // column c+1 first, one row per thread: the next search reads it
             for (int r = rs1 + t; r < rows; r += PNT)
                 tile[r * LDT + c + 1] = tile[r * LDT + c + 1] + lcol[r] * u1;
+            __syncthreads();
+            next = search(c + 1);
         }
+        // ---- the fetched rows: the pivot row into urow and position k, row k
+        // into position p (its columns c and c+1 are already in place)
+        bool fail = false;
+#pragma unroll
+        for (int i = 0; i < NF; ++i) {
+            const int j = t + i * PNT;
+            if (j > B) continue;
+            long spins = 0;
+            while (word_tag(fw[i]) != tag && !fail) {
+                if (++spins > spin_limit) fail = true;
+                __builtin_amdgcn_s_sleep(1);
+                fw[i] = get_tagged(wrow + j);
+            }
+            if (own_p)
+                while (word_tag(fk[i]) != tag && !fail) {
+                    if (++spins > spin_limit) fail = true;
+                    __builtin_amdgcn_s_sleep(1);
+                    fk[i] = get_tagged(krw + j);
+                }
+            const float uj = __uint_as_float(word_payload(fw[i]));
+            urow[j] = uj;
+            if (own_k) tile[rk * LDT + j] = uj;
+            // (at the last column, c + 1 == B is the row-of-origin word)
+            if (own_p && j != c && (j != c + 1 || j == B))
+                tile[rp * LDT + j] = __uint_as_float(word_payload(fk[i]));
+        }
+        if (fail) abort_flag = 1;
         __syncthreads();
-        search_arrive(c + 1, true);
-        // the other columns, one column per TPC threads
+        if (abort_flag) {
+            give_up();
+            return;
+        }
+        if (c + 1 == B) break;
+        publish(c + 1, next, true);
+        __syncthreads();  // the published rows were read before they change
+        // the other columns, one column per TPC threads, 16 rows in flight
         const int j = t % B;
         if (j > c + 1) {
             const float uj = urow[j];
             int r = rs1 + t / B;
-            for (; r + 7 * TPC < rows; r += 8 * TPC) {
-                float v[8], lv[8];
+            for (; r + 15 * TPC < rows; r += 16 * TPC) {
+                float v[16], lv[16];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) {
+                for (int u = 0; u < 16; ++u) {
                     v[u] = tile[(r + u * TPC) * LDT + j];
                     lv[u] = lcol[r + u * TPC];
                 }
 #pragma unroll
-                for (int u = 0; u < 8; ++u) tile[(r + u * TPC) * LDT + j] = v[u] + lv[u] * uj;
+                for (int u = 0; u < 16; ++u) tile[(r + u * TPC) * LDT + j] = v[u] + lv[u] * uj;
             }
             for (; r < rows; r += TPC) tile[r * LDT + j] = tile[r * LDT + j] + lcol[r] * uj;
         }
@@ -319,9 +471,10 @@ __global__ __launch_bounds__(PNT) void panel_pivot_kernel(
     for (int r = t; r < rows; r += PNT) src[p0 + r] = __float_as_int(tile[r * LDT + B]);
 }
 
-__global__ void panel_reset_kernel(unsigned long long* keys, int B, unsigned* ctr) {
-    for (int i = threadIdx.x; i < B; i += blockDim.x) keys[i] = 0ULL;
-    if (threadIdx.x == 0) *ctr = 0u;
+__global__ void zero_words_kernel(unsigned long long* __restrict__ p, size_t words) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < words;
+         i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0ull;
 }
 
 // ---------------------------------------------------------------- exchanges
@@ -466,10 +619,16 @@ hipError_t launch_transpose_square(float* A, int64_t ld, int64_t n, hipStream_t 
 
 int panel_pivot_rows(int B) { return B >= 256 ? 128 : 256; }
 
+namespace {
 size_t panel_pivot_lds(int B) {
     const int R = panel_pivot_rows(B);
     return ((size_t)R * (B + 1) + (B + 1) + R) * sizeof(float);
 }
+// Dynamic LDS requested at least, so that a CU holds at most one panel
+// workgroup: a grid no larger than the CU count then gets a CU per workgroup
+// instead of two workgroups sharing one.
+constexpr size_t kSpreadLds = 96 * 1024;
+}  // namespace
 
 int panel_pivot_groups(int64_t m, int B) {
     const int R = panel_pivot_rows(B);
@@ -486,7 +645,8 @@ hipError_t panel_pivot_prepare(int B, int* max_groups) {
         case 256: k = reinterpret_cast<const void*>(panel_pivot_kernel<256>); break;
         default: return hipErrorInvalidValue;
     }
-    hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)std::max(lds, kSpreadLds));
     if (e != hipSuccess) return e;
     int per_cu = 0;
     e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, PNT, lds);
@@ -496,32 +656,36 @@ hipError_t panel_pivot_prepare(int B, int* max_groups) {
     if (e != hipSuccess) return e;
     e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     if (e != hipSuccess) return e;
-    *max_groups = per_cu * cus;
+    *max_groups = std::min(per_cu * cus, kMaxPanelGroups - 1);
     return hipSuccess;
 }
 
-hipError_t launch_panel_pivot(float* X, int64_t ldx, int64_t n, int64_t k0, int B, float* pub,
-                              float* krow, unsigned long long* keys, unsigned* ctr, float* Lt,
-                              int64_t ldl, float* diagU, int* ipvt, int* src, int* status,
-                              int max_groups, hipStream_t s, long spin_limit) {
+hipError_t launch_panel_pivot(float* X, int64_t ldx, int64_t n, int64_t k0, int B,
+                              unsigned long long* pub, unsigned long long* krow,
+                              unsigned long long* kslot, int gslots, float* Lt, int64_t ldl,
+                              float* diagU, int* ipvt, int* src, int* status, int max_groups,
+                              int num_cu, hipStream_t s, long spin_limit) {
     const int R = panel_pivot_rows(B);
     // spin_limit < 0: give up after -spin_limit polls, waiting for one more
-    // workgroup than the grid holds, so that every barrier gives up (the
+    // workgroup than the grid holds, so that every exchange gives up (the
     // error path, exercised from the host with every kernel exiting)
     const unsigned phantom = spin_limit < 0 ? 1u : 0u;
     spin_limit = spin_limit == 0 ? kSpinLimit : (spin_limit < 0 ? -spin_limit : spin_limit);
     const int64_t m = n - k0;
     const int G = panel_pivot_groups(m, B);
-    if (G < 1 || G > max_groups) return hipErrorInvalidConfiguration;
-    // 16-byte row vectors
-    if ((ldx & 3) || (k0 & 3) || (reinterpret_cast<uintptr_t>(X) & 15)) return hipErrorInvalidValue;
-    panel_reset_kernel<<<1, 256, 0, s>>>(keys, B, ctr);
-    const size_t lds = panel_pivot_lds(B);
-#define HPL_PANEL(BB)                                                                           \
-    case BB:                                                                                    \
-        panel_pivot_kernel<BB><<<G, PNT, lds, s>>>(X, ldx, n, k0, R, pub, krow, keys, ctr, Lt, \
-                                                   ldl, diagU, ipvt, src, status, spin_limit,   \
-                                                   phantom);                                    \
+    if (G < 1 || G > max_groups || G + (int)phantom > gslots ||
+        G + (int)phantom > kMaxPanelGroups)
+        return hipErrorInvalidConfiguration;
+    // 16-byte row vectors; tags are positions + 1 and must stay below the abort tag
+    if ((ldx & 3) || (k0 & 3) || (reinterpret_cast<uintptr_t>(X) & 15) || n >= (1LL << 31))
+        return hipErrorInvalidValue;
+    size_t lds = panel_pivot_lds(B);
+    if (G <= num_cu) lds = std::max(lds, kSpreadLds);
+#define HPL_PANEL(BB)                                                                        \
+    case BB:                                                                                 \
+        panel_pivot_kernel<BB><<<G, PNT, lds, s>>>(X, ldx, n, k0, R, pub, krow, kslot, gslots, \
+                                                   Lt, ldl, diagU, ipvt, src, status,          \
+                                                   spin_limit, phantom);                       \
         break;
     switch (B) {
         HPL_PANEL(32)
@@ -531,6 +695,18 @@ hipError_t launch_panel_pivot(float* X, int64_t ldx, int64_t n, int64_t k0, int 
         default: return hipErrorInvalidValue;
     }
 #undef HPL_PANEL
+    return hipGetLastError();
+}
+
+size_t panel_exchange_words(int64_t n, int B) {
+    const int64_t groups = panel_pivot_groups(n, B);
+    return 2 * (size_t)groups * (B + 1) + 2 * (size_t)(B + 1) +
+           2 * (size_t)(groups + 1) * kSlotWords;
+}
+
+hipError_t launch_zero_words(unsigned long long* p, size_t words, hipStream_t s) {
+    const unsigned blocks = (unsigned)std::min<size_t>((words + 255) / 256, 1024);
+    if (blocks) zero_words_kernel<<<blocks, 256, 0, s>>>(p, words);
     return hipGetLastError();
 }
 
