@@ -37,7 +37,7 @@ using hplx::fail;
 
 // Byte offsets inside the context's pivot block.
 struct PvLayout {
-    size_t piv, perm, src, diag, xw, list, y, d, total;
+    size_t piv, perm, src, diag, xw, list, y, d, lt2, total;
 };
 
 size_t al256(size_t x) { return (x + 255) / 256 * 256; }
@@ -58,6 +58,7 @@ PvLayout pv_layout(uint64_t n, uint32_t B) {
     L.list = take((4 * (size_t)B + 1) * sizeof(int));
     L.y = take(2 * (size_t)B * n * sizeof(float));
     L.d = take((size_t)B * B * sizeof(float));
+    L.lt2 = take((size_t)B * n * sizeof(float));
     L.total = o;
     return L;
 }
@@ -128,11 +129,24 @@ hpl_status gefa_pivot(hpl_ctx* c, uint32_t B) {
     PCHK(c, launch_zero_words(pub, panel_exchange_words(n, (int)B), s));
     PCHK(c, launch_transpose_square(c->dA, lda, n, s));
     PCHK(c, launch_iota(c->dPerm, n, s));
+    // Look-ahead: step k updates the next panel's B columns on s0 first; the
+    // rest of its trailing update runs on s1 while panel k+1 factorises on s0
+    // (on the CUs the update leaves).  Panel k+1's row moves touch every
+    // column, so they wait for that update.  The panels' transposed
+    // multipliers alternate between two buffers: panel k+1 writes its own
+    // while step k's update still reads panel k's.
+    float* lts[2] = {c->dLt, reinterpret_cast<float*>(base + L.lt2)};
+    bool rest_pending = false;
     for (int64_t k = 0; k < N; ++k) {
         const int64_t k0 = k * B, m1 = n - k0 - B;
+        float* lt = lts[k & 1];
         PCHK(c, launch_panel_pivot(c->dA, lda, n, k0, (int)B, pub, krow, kslot, (int)groups + 1,
-                                   c->dLt, n, c->dDiagU, c->dPiv, src, c->dStatus,
-                                   c->pv_max_groups, c->num_cu, s, c->pv_spin_limit));
+                                   lt, n, c->dDiagU, c->dPiv, src, c->dStatus, c->pv_max_groups,
+                                   c->num_cu, s, c->pv_spin_limit));
+        if (rest_pending) {
+            PCHK(c, hipStreamWaitEvent(s, c->evP, 0));
+            rest_pending = false;
+        }
         PCHK(c, launch_row_moves(c->dA, lda, n, k0, (int)B, c->dPiv, src, list, count, Y, n,
                                  c->dPerm, c->dStatus, s));
         if (m1 > 0) {
@@ -140,11 +154,22 @@ hpl_status gefa_pivot(hpl_ctx* c, uint32_t B) {
             PCHK(c, launch_unit_lower(c->dA, lda, k0, (int)B, D, s));
             PCHK(c, launch_prep(D, B, 0, (int)B, c->dDT, inv, s));
             PCHK(c, launch_trsm_top(c->dA, lda, k0, m1, (int)B, c->dDT, inv, s));
-            PCHK(c, launch_gemm_update_f32(c->dA + (k0 + B) * lda + k0 + B, lda, c->dLt + k0 + B,
-                                           n, c->dA + k0 * lda + k0 + B, lda, (int)m1, (int)m1,
+            float* C = c->dA + (k0 + B) * lda + k0 + B;
+            const float* T = c->dA + k0 * lda + k0 + B;
+            const int64_t nb = std::min<int64_t>(m1, B);
+            PCHK(c, launch_gemm_update_f32(C, lda, lt + k0 + B, n, T, lda, (int)m1, (int)nb,
                                            (int)B, s));
+            if (m1 > nb) {
+                PCHK(c, hipEventRecord(c->evA, s));
+                PCHK(c, hipStreamWaitEvent(c->s1, c->evA, 0));
+                PCHK(c, launch_gemm_update_f32(C + nb, lda, lt + k0 + B, n, T + nb, lda, (int)m1,
+                                               (int)(m1 - nb), (int)B, c->s1));
+                PCHK(c, hipEventRecord(c->evP, c->s1));
+                rest_pending = true;
+            }
         }
     }
+    if (rest_pending) PCHK(c, hipStreamWaitEvent(s, c->evP, 0));
     PCHK(c, launch_transpose_square(c->dA, lda, n, s));
     // the substitution's 32 x 32 inverses, from the final factor
     PCHK(c, launch_prep_all(c->dA, lda, (int)B, (int)N, c->dInv, inv_stride, s));
