@@ -1,10 +1,14 @@
 // C ABI for batched solves of many systems of uneven order (hpl_batch_*).
 //
 // Systems are grouped in size buckets by their padded order (<= 128, 256, 512,
-// 1024).  Each bucket is one persistent work-queue launch whose workgroup size
-// the autotuner picks by timing; the buckets' launches run concurrently on
-// forked streams and the whole solve -- restore of the matrices, queue resets,
-// the bucket launches -- is captured once into a HIP graph and replayed.
+// 1024).  A bucket is solved either by one persistent work-queue launch (one
+// workgroup per system) or blocked: one launch per 32-wide step for the
+// diagonal blocks, the strips and the trailing updates of all its systems,
+// then the persistent kernel for the substitutions and refinement only.  The
+// autotuner picks the mode and the workgroup size by timing; the buckets'
+// launches run concurrently on forked streams and the whole solve -- restore
+// of the matrices, queue resets, the bucket launches -- is captured once into
+// a HIP graph and replayed.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -28,6 +32,12 @@ struct Bucket {
     int count = 0;
     int maxm = 0;       // largest trailing size (npad - 32)
     int threads = 256;  // workgroup size (autotuned)
+    bool blocked = false;  // blocked factorisation (autotuned)
+    // scratch of the blocked factorisation: diagonal blocks (count x 1056
+    // floats), strip copies (count x maxm x 32 floats each)
+    float* dDg = nullptr;
+    void* dLg = nullptr;
+    void* dTg = nullptr;
 };
 
 }  // namespace
@@ -50,6 +60,7 @@ struct hpl_batch {
     double* drs = nullptr;
     float* dxs = nullptr;
     BatchResult* dRes = nullptr;
+    int* dBad = nullptr;  // zero-pivot flags of the blocked factorisation
     hipStream_t s0 = nullptr;
     hipStream_t sb[kBuckets] = {};
     hipEvent_t fork = nullptr, join[kBuckets] = {}, t0 = nullptr, t1 = nullptr;
@@ -97,13 +108,49 @@ int grid_for(const hpl_batch* b, const Bucket& k, bool f16) {
     return std::max(1, std::min(k.count, b->cus * per_cu));
 }
 
+// the blocked scratch of bucket k, allocated on first use (never under capture)
+hpl_status ensure_blocked(hpl_batch* b, Bucket& k) {
+    if (k.dDg || !k.count) return HPL_SUCCESS;
+    const size_t strip = (size_t)k.count * std::max(k.maxm, 32) * 32 * sizeof(float);
+    if (!b->dBad) BCHK(b, hipMalloc(&b->dBad, b->nsys * sizeof(int)));
+    BCHK(b, hipMalloc(&k.dDg, (size_t)k.count * 1056 * sizeof(float)));
+    BCHK(b, hipMalloc(&k.dLg, strip));
+    BCHK(b, hipMalloc(&k.dTg, strip));
+    return HPL_SUCCESS;
+}
+
 hpl_status enqueue_bucket(hpl_batch* b, int i, bool f16, int max_iter, double tol,
                           hipStream_t s) {
     const Bucket& k = b->buckets[i];
     if (!k.count) return HPL_SUCCESS;
-    BCHK(b, launch_batch_solve(k.threads, f16, grid_for(b, k, f16), b->dSys, b->dOrder + k.begin,
-                               k.count, b->dCounters + i, b->dA, b->dA0, b->db64, b->dx64,
-                               b->dxs, b->drs, max_iter, tol, b->dRes, std::max(k.maxm, 32), s));
+    if (!k.blocked) {
+        BCHK(b, launch_batch_solve(k.threads, f16, grid_for(b, k, f16), b->dSys,
+                                   b->dOrder + k.begin, k.count, b->dCounters + i, b->dA, b->dA0,
+                                   b->db64, b->dx64, b->dxs, b->drs, max_iter, tol, b->dRes,
+                                   std::max(k.maxm, 32), s));
+        return HPL_SUCCESS;
+    }
+    if (!k.dDg) return bfail(b, HPL_ERR_STATE, "blocked scratch of bucket %d missing", i);
+    // steps over the systems still longer than the step: a prefix of the
+    // bucket's largest-first order
+    const int* order = b->order.data() + k.begin;
+    const int top = b->sys[order[0]].npad;
+    const size_t sstride = (size_t)std::max(k.maxm, 32) * 32;
+    for (int k0 = 0; k0 < top; k0 += 32) {
+        int act_diag = 0, act = 0;
+        while (act_diag < k.count && b->sys[order[act_diag]].npad > k0) ++act_diag;
+        while (act < k.count && b->sys[order[act]].npad > k0 + 32) ++act;
+        const int mmax = act ? top - k0 - 32 : 0;
+        BCHK(b, launch_batch_step(f16, b->dSys, b->dOrder + k.begin, k0, act_diag, act, mmax,
+                                  b->dA, k.dDg, b->dBad, k.dLg, k.dTg, sstride, s));
+    }
+    // substitutions and refinement: LDS for npad doubles and npad floats
+    Bucket solve = k;
+    solve.maxm = std::max(32, (k.hi + 4) / 5);
+    BCHK(b, launch_batch_solve(k.threads, f16, grid_for(b, solve, f16), b->dSys,
+                               b->dOrder + k.begin, k.count, b->dCounters + i, b->dA, b->dA0,
+                               b->db64, b->dx64, b->dxs, b->drs, max_iter, tol, b->dRes,
+                               solve.maxm, s, b->dBad));
     return HPL_SUCCESS;
 }
 
@@ -215,8 +262,11 @@ void hpl_batch_destroy(hpl_batch* b) {
     drop_graph(b);
     for (void* p : {(void*)b->dSys, (void*)b->dOrder, (void*)b->dCounters, (void*)b->dA,
                     (void*)b->dA0, (void*)b->db64, (void*)b->dx64, (void*)b->drs, (void*)b->dxs,
-                    (void*)b->dRes})
+                    (void*)b->dRes, (void*)b->dBad})
         if (p) (void)hipFree(p);
+    for (auto& k : b->buckets)
+        for (void* p : {(void*)k.dDg, k.dLg, k.dTg})
+            if (p) (void)hipFree(p);
     for (int i = 0; i < kBuckets; ++i) {
         if (b->sb[i]) (void)hipStreamDestroy(b->sb[i]);
         if (b->join[i]) (void)hipEventDestroy(b->join[i]);
@@ -256,6 +306,19 @@ hpl_status hpl_batch_set_system(hpl_batch* b, int s, const float* a, uint64_t ld
     return HPL_SUCCESS;
 }
 
+hpl_status hpl_batch_set_blocked(hpl_batch* b, int blocked) {
+    if (!b) return HPL_ERR_INVALID_ARG;
+    drop_graph(b);
+    for (auto& k : b->buckets) {
+        if (blocked) {
+            hpl_status st = ensure_blocked(b, k);
+            if (st) return st;
+        }
+        k.blocked = blocked != 0 && k.count > 0;
+    }
+    return HPL_SUCCESS;
+}
+
 hpl_status hpl_batch_autotune(hpl_batch* b, hpl_prec prec, int max_iter, double tol_per_n,
                               int* threads_out) {
     if (!b) return HPL_ERR_INVALID_ARG;
@@ -263,15 +326,22 @@ hpl_status hpl_batch_autotune(hpl_batch* b, hpl_prec prec, int max_iter, double 
         return bfail(b, HPL_ERR_UNSUPPORTED, "batched solves run FP16 or FP32 trailing updates");
     const bool f16 = prec == HPL_PREC_FP16;
     drop_graph(b);
-    const int cands[2] = {128, 256};
+    const int cands[3] = {128, 256, 512};
     for (int i = 0; i < kBuckets; ++i) {
         Bucket& k = b->buckets[i];
         if (!k.count) continue;
-        if (!f16 && batch_lds_bytes(std::max(k.maxm, 32), false) > kBatchLdsLimit) continue;
+        hpl_status st0 = ensure_blocked(b, k);
+        if (st0) return st0;
+        const bool lds_ok = f16 || batch_lds_bytes(std::max(k.maxm, 32), false) <= kBatchLdsLimit;
         float best = 1e30f;
         int best_t = k.threads;
-        for (int t : cands) {
+        bool best_blk = k.blocked;
+        for (int cand = 0; cand < 6; ++cand) {
+            const int t = cands[cand % 3];
+            const bool blk = cand >= 3;
+            if (!blk && !lds_ok) continue;
             k.threads = t;
+            k.blocked = blk;
             float ms_sum = 0.f;
             for (int rep = 0; rep < 2; ++rep) {
                 BCHK(b, hipMemcpyAsync(b->dA, b->dA0, b->a_total * sizeof(float),
@@ -288,12 +358,17 @@ hpl_status hpl_batch_autotune(hpl_batch* b, hpl_prec prec, int max_iter, double 
             if (ms_sum < best) {
                 best = ms_sum;
                 best_t = t;
+                best_blk = blk;
             }
         }
         k.threads = best_t;
+        k.blocked = best_blk;
     }
     if (threads_out)
-        for (int i = 0; i < kBuckets; ++i) threads_out[i] = b->buckets[i].count ? b->buckets[i].threads : 0;
+        for (int i = 0; i < kBuckets; ++i) {
+            const Bucket& k = b->buckets[i];
+            threads_out[i] = k.count ? (k.blocked ? -k.threads : k.threads) : 0;
+        }
     return HPL_SUCCESS;
 }
 
@@ -305,8 +380,10 @@ hpl_status hpl_batch_solve(hpl_batch* b, hpl_prec prec, int max_iter, double tol
     const bool f16 = prec == HPL_PREC_FP16;
     if (!f16)
         for (auto& k : b->buckets)
-            if (k.count && batch_lds_bytes(std::max(k.maxm, 32), false) > kBatchLdsLimit)
-                return bfail(b, HPL_ERR_UNSUPPORTED, "FP32 batched updates need n <= 480");
+            if (k.count && !k.blocked &&
+                batch_lds_bytes(std::max(k.maxm, 32), false) > kBatchLdsLimit)
+                return bfail(b, HPL_ERR_UNSUPPORTED,
+                             "FP32 batched updates need n <= 480 (or the blocked factorisation)");
     if (!b->exec || b->graph_prec != (int)prec || b->graph_iter != max_iter ||
         b->graph_tol != tol_per_n) {
         drop_graph(b);

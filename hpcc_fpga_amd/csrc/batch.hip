@@ -50,7 +50,8 @@ __device__ __forceinline__ double block_max(double v, double* red) {
 }
 
 // Solve M x = rhs in place on x (FP32, in LDS), factor in A (npad, dgefa
-// encoding).  The GEMV loops keep all 32 factor loads of a block in flight.
+// encoding).  The diagonal blocks' solves and the GEMV loops keep all 32
+// factor loads of a block in flight.
 template <int NT>
 __device__ __forceinline__ void substitute(const float* __restrict__ A, int npad,
                                            float* __restrict__ x) {
@@ -58,10 +59,16 @@ __device__ __forceinline__ void substitute(const float* __restrict__ A, int npad
     // forward: x_e += sum_p A[p][e] x_p, block by block
     for (int k0 = 0; k0 < npad; k0 += 32) {
         if (wave == 0 && lane < 32) {
+            // the block's column `lane`, all 32 loads in flight before the
+            // dependent chain
+            float d[32];
+#pragma unroll
+            for (int p = 0; p < 32; ++p) d[p] = A[(int64_t)(k0 + p) * npad + k0 + lane];
             float xi = x[k0 + lane];
+#pragma unroll
             for (int p = 0; p < 32; ++p) {
                 const float xp = __shfl(xi, p);
-                if (lane > p) xi += xp * A[(int64_t)(k0 + p) * npad + k0 + lane];
+                if (lane > p) xi += xp * d[p];
             }
             x[k0 + lane] = xi;
         }
@@ -77,11 +84,18 @@ __device__ __forceinline__ void substitute(const float* __restrict__ A, int npad
     // backward: x_k = x_k / u_kk (stored -1/u_kk), x_i -= x_k u_ik (i < k)
     for (int k0 = npad - 32; k0 >= 0; k0 -= 32) {
         if (wave == 0 && lane < 32) {
+            float d[32], dg[32];
+#pragma unroll
+            for (int p = 0; p < 32; ++p) {
+                d[p] = A[(int64_t)(k0 + p) * npad + k0 + lane];
+                dg[p] = A[(int64_t)(k0 + p) * npad + k0 + p];
+            }
             float xi = x[k0 + lane];
+#pragma unroll
             for (int p = 31; p >= 0; --p) {
-                float sc = __shfl(xi, p) * A[(int64_t)(k0 + p) * npad + k0 + p];
+                float sc = __shfl(xi, p) * dg[p];
                 if (lane == p) xi = -sc;
-                if (lane < p) xi += sc * A[(int64_t)(k0 + p) * npad + k0 + lane];
+                if (lane < p) xi += sc * d[p];
             }
             x[k0 + lane] = xi;
         }
@@ -104,7 +118,7 @@ __global__ __launch_bounds__(NT) void batch_solve_kernel(
     int* __restrict__ counter, float* __restrict__ A, const float* __restrict__ A0,
     const double* __restrict__ b64, double* __restrict__ x64, float* __restrict__ xs,
     double* __restrict__ rs, int max_iter, double tol_per_n, BatchResult* __restrict__ out,
-    int maxm) {
+    int maxm, const int* __restrict__ pre_bad) {
     typedef typename std::conditional<F16, _Float16, float>::type T;
     extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
     constexpr int LDT = 40;  // padded row of the strip copies (elements)
@@ -129,7 +143,10 @@ __global__ __launch_bounds__(NT) void batch_solve_kernel(
         bool bad = false;
 
         // ------------------------------------------------------------ factor
-        for (int k0 = 0; k0 < npad; k0 += 32) {
+        // (pre_bad: factorised by the blocked kernels below, which flagged
+        // zero or non-finite pivots there)
+        if (pre_bad) bad = pre_bad[s] != 0;
+        for (int k0 = 0; k0 < (pre_bad ? 0 : npad); k0 += 32) {
             if (wave == 0) {
                 // diagonal block: lane c < 32 holds column k0+c, rows 0..31
                 float d[32];
@@ -317,6 +334,159 @@ __global__ __launch_bounds__(NT) void batch_solve_kernel(
     }
 }
 
+// ---------------------------------------------------------------- blocked
+// The same factorisation, one 32-wide step at a time over every system of a
+// bucket still longer than the step: the first `act` systems of its
+// largest-first order.  One launch factorises the diagonal blocks, one solves
+// the strips, one applies the trailing updates, so a large system's step
+// spreads over many CUs instead of one workgroup.  Each element sees the
+// persistent kernel's operations in the same order, so the factors are
+// bit-identical.  The strips' copies (type T) go to global scratch, row
+// stride 32, at Lg / Tg + j * sstride for the j-th active system; its
+// diagonal block to Dg + j * 1056 (32 x 33).
+
+// one wave per system: the diagonal block of step k0
+__global__ __launch_bounds__(64) void bstep_diag_kernel(const BatchSystem* __restrict__ sys,
+                                                        const int* __restrict__ order, int k0,
+                                                        float* __restrict__ A,
+                                                        float* __restrict__ Dg,
+                                                        int* __restrict__ bad_out) {
+    const int j = blockIdx.x, s = order[j];
+    const int npad = sys[s].npad;
+    float* a = A + sys[s].a_off;
+    const int lane = threadIdx.x, lr = lane & 31;
+    bool bad = false;
+    float d[32];
+#pragma unroll
+    for (int r = 0; r < 32; ++r) d[r] = lr == lane ? a[(int64_t)(k0 + r) * npad + k0 + lr] : 0.0f;
+#pragma unroll
+    for (int jj = 0; jj < 32; ++jj) {
+        const float piv = bcast(d[jj], jj);
+        const float inv = -1.0f / piv;
+        bad |= (piv == 0.0f) || !__builtin_isfinite(inv);
+        d[jj] = (lane > jj) ? __fmul_rn(d[jj], inv) : ((lane == jj) ? inv : d[jj]);
+#pragma unroll
+        for (int r = jj + 1; r < 32; ++r) {
+            const float sv = bcast(d[r], jj);
+            if (lane > jj) d[r] = __fadd_rn(d[r], __fmul_rn(d[jj], sv));
+        }
+    }
+    if (lane < 32) {
+#pragma unroll
+        for (int r = 0; r < 32; ++r) {
+            a[(int64_t)(k0 + r) * npad + k0 + lane] = d[r];
+            Dg[(size_t)j * 1056 + r * 33 + lane] = d[r];
+        }
+    }
+    // the first step sets the flag, later ones only raise it
+    if (lane == 0 && (k0 == 0 || bad)) bad_out[s] = bad ? 1 : 0;
+}
+
+// the strips of step k0: 256 columns of the top strip and 256 rows of the
+// left strip per workgroup
+template <bool F16>
+__global__ __launch_bounds__(256) void bstep_strips_kernel(const BatchSystem* __restrict__ sys,
+                                                           const int* __restrict__ order, int k0,
+                                                           float* __restrict__ A,
+                                                           const float* __restrict__ Dg,
+                                                           void* __restrict__ Lv,
+                                                           void* __restrict__ Tv,
+                                                           size_t sstride) {
+    typedef typename std::conditional<F16, _Float16, float>::type T;
+    const int j = blockIdx.y, s = order[j];
+    const int npad = sys[s].npad, r0 = k0 + 32, m = npad - r0;
+    if ((int)blockIdx.x * 256 >= m) return;  // the whole workgroup, before any barrier
+    __shared__ float D[32 * 33];
+    for (int e = threadIdx.x; e < 32 * 33; e += 256) D[e] = Dg[(size_t)j * 1056 + e];
+    __syncthreads();
+    float* a = A + sys[s].a_off;
+    T* Left = static_cast<T*>(Lv) + (size_t)j * sstride;
+    T* TopT = static_cast<T*>(Tv) + (size_t)j * sstride;
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= m) return;
+    {  // top strip: column r0 + c of rows k0..k0+31 (closed form top)
+        float x[32];
+#pragma unroll
+        for (int i = 0; i < 32; ++i) x[i] = a[(int64_t)(k0 + i) * npad + r0 + c];
+#pragma unroll
+        for (int kk = 0; kk < 32; ++kk) {
+            x[kk] = __fmul_rn(x[kk], D[kk * 33 + kk]);
+#pragma unroll
+            for (int jj = kk + 1; jj < 32; ++jj) x[jj] = __fadd_rn(x[jj], __fmul_rn(x[kk], D[jj * 33 + kk]));
+        }
+#pragma unroll
+        for (int i = 0; i < 32; ++i) {
+            a[(int64_t)(k0 + i) * npad + r0 + c] = x[i];
+            TopT[(size_t)c * 32 + i] = (T)x[i];
+        }
+    }
+    {  // left strip: row r0 + c, columns k0..k0+31 (closed form left)
+        float x[32];
+        float* row = a + (int64_t)(r0 + c) * npad + k0;
+#pragma unroll
+        for (int i = 0; i < 32; i += 4) {
+            const f32x4 v = *reinterpret_cast<const f32x4*>(row + i);
+            x[i] = v[0];
+            x[i + 1] = v[1];
+            x[i + 2] = v[2];
+            x[i + 3] = v[3];
+        }
+#pragma unroll
+        for (int kk = 0; kk < 32; ++kk)
+#pragma unroll
+            for (int i = kk + 1; i < 32; ++i) x[i] = __fadd_rn(x[i], __fmul_rn(x[kk], D[kk * 33 + i]));
+#pragma unroll
+        for (int i = 0; i < 32; i += 4)
+            *reinterpret_cast<f32x4*>(row + i) = f32x4{x[i], x[i + 1], x[i + 2], x[i + 3]};
+#pragma unroll
+        for (int i = 0; i < 32; ++i) Left[(size_t)c * 32 + i] = (T)x[i];
+    }
+}
+
+// the trailing update of step k0: one 32 x 32 tile per wave
+template <bool F16>
+__global__ __launch_bounds__(256) void bstep_update_kernel(const BatchSystem* __restrict__ sys,
+                                                           const int* __restrict__ order, int k0,
+                                                           float* __restrict__ A,
+                                                           const void* __restrict__ Lv,
+                                                           const void* __restrict__ Tv,
+                                                           size_t sstride) {
+    typedef typename std::conditional<F16, _Float16, float>::type T;
+    const int j = blockIdx.y, s = order[j];
+    const int npad = sys[s].npad, r0 = k0 + 32, m = npad - r0;
+    const int tiles = m / 32, ntile = tiles * tiles;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int tt = blockIdx.x * 4 + wave;
+    if (tt >= ntile) return;  // wave-uniform; no barrier in this kernel
+    const int lr = lane & 31, lh = lane >> 5;
+    const int ti = tt / tiles, tj = tt % tiles;
+    const T* Left = static_cast<const T*>(Lv) + (size_t)j * sstride;
+    const T* TopT = static_cast<const T*>(Tv) + (size_t)j * sstride;
+    float* cbase = A + sys[s].a_off + (int64_t)(r0 + ti * 32) * npad + r0 + tj * 32;
+    f32x16 acc;
+#pragma unroll
+    for (int rr = 0; rr < 16; ++rr) acc[rr] = cbase[(int64_t)acc_row(rr, lane) * npad + lr];
+    if constexpr (F16) {
+#pragma unroll
+        for (int ks = 0; ks < 2; ++ks) {
+            const f16x8 av =
+                *reinterpret_cast<const f16x8*>(&Left[(size_t)(ti * 32 + lr) * 32 + ks * 16 + 8 * lh]);
+            const f16x8 bv =
+                *reinterpret_cast<const f16x8*>(&TopT[(size_t)(tj * 32 + lr) * 32 + ks * 16 + 8 * lh]);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(av, bv, acc, 0, 0, 0);
+        }
+    } else {
+#pragma unroll
+        for (int kp = 0; kp < 16; ++kp) {
+            const float av = Left[(size_t)(ti * 32 + lr) * 32 + 2 * kp + lh];
+            const float bv = TopT[(size_t)(tj * 32 + lr) * 32 + 2 * kp + lh];
+            acc = mfma32(av, bv, acc);
+        }
+    }
+#pragma unroll
+    for (int rr = 0; rr < 16; ++rr) cbase[(int64_t)acc_row(rr, lane) * npad + lr] = acc[rr];
+}
+
 // Generator: system s, row-major npad x npad; M(i,j) = U[-0.5,0.5) for
 // i, j < n (counter-based, key of (seed, s)), M(i,i) += n, identity on the
 // pad; b = M * 1 in FP64 on the first n entries, 0 on the pad.
@@ -357,7 +527,9 @@ hipError_t batch_prepare() {
     const void* ks[] = {reinterpret_cast<const void*>(batch_solve_kernel<128, true>),
                         reinterpret_cast<const void*>(batch_solve_kernel<128, false>),
                         reinterpret_cast<const void*>(batch_solve_kernel<256, true>),
-                        reinterpret_cast<const void*>(batch_solve_kernel<256, false>)};
+                        reinterpret_cast<const void*>(batch_solve_kernel<256, false>),
+                        reinterpret_cast<const void*>(batch_solve_kernel<512, true>),
+                        reinterpret_cast<const void*>(batch_solve_kernel<512, false>)};
     for (const void* k : ks) {
         hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
         if (e != hipSuccess) return e;
@@ -376,17 +548,42 @@ hipError_t launch_batch_generate(const BatchSystem* sys, int nsys, uint64_t seed
     return hipGetLastError();
 }
 
+hipError_t launch_batch_step(bool f16, const BatchSystem* sys, const int* order, int k0,
+                             int act_diag, int act, int mmax, float* A, float* Dg, int* bad,
+                             void* Lg, void* Tg, size_t sstride, hipStream_t s) {
+    if (act_diag < 1 || act > act_diag || act > 65535) return hipErrorInvalidValue;
+    bstep_diag_kernel<<<act_diag, 64, 0, s>>>(sys, order, k0, A, Dg, bad);
+    if (act > 0 && mmax > 0) {
+        if (mmax % 32 || (size_t)mmax * 32 > sstride) return hipErrorInvalidValue;
+        const dim3 gs((unsigned)((mmax + 255) / 256), (unsigned)act);
+        const int tiles = mmax / 32;
+        const dim3 gu((unsigned)((tiles * tiles + 3) / 4), (unsigned)act);
+        if (f16) {
+            bstep_strips_kernel<true><<<gs, 256, 0, s>>>(sys, order, k0, A, Dg, Lg, Tg, sstride);
+            bstep_update_kernel<true><<<gu, 256, 0, s>>>(sys, order, k0, A, Lg, Tg, sstride);
+        } else {
+            bstep_strips_kernel<false><<<gs, 256, 0, s>>>(sys, order, k0, A, Dg, Lg, Tg, sstride);
+            bstep_update_kernel<false><<<gu, 256, 0, s>>>(sys, order, k0, A, Lg, Tg, sstride);
+        }
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_batch_solve(int threads, bool f16, int grid, const BatchSystem* sys,
                               const int* order, int nsys, int* counter, float* A, const float* A0,
                               const double* b64, double* x64, float* xs, double* rs, int max_iter,
-                              double tol_per_n, BatchResult* out, int maxm, hipStream_t s) {
+                              double tol_per_n, BatchResult* out, int maxm, hipStream_t s,
+                              const int* pre_bad) {
     const size_t lds = batch_lds_bytes(maxm, f16);
     if (lds > kBatchLdsLimit) return hipErrorInvalidValue;
     reset_counter_kernel<<<1, 64, 0, s>>>(counter);
 #define HPL_BATCH(NTV, F)                                                                     \
     batch_solve_kernel<NTV, F><<<grid, NTV, lds, s>>>(sys, order, nsys, counter, A, A0, b64, x64, \
-                                                     xs, rs, max_iter, tol_per_n, out, maxm)
-    if (threads == 256) {
+                                                     xs, rs, max_iter, tol_per_n, out, maxm,  \
+                                                     pre_bad)
+    if (threads == 512) {
+        if (f16) HPL_BATCH(512, true); else HPL_BATCH(512, false);
+    } else if (threads == 256) {
         if (f16) HPL_BATCH(256, true); else HPL_BATCH(256, false);
     } else if (threads == 128) {
         if (f16) HPL_BATCH(128, true); else HPL_BATCH(128, false);

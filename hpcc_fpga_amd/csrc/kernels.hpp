@@ -245,7 +245,20 @@ hipError_t batch_prepare();  // allow kBatchLdsLimit bytes of dynamic LDS
 size_t batch_lds_bytes(int maxm, bool f16);
 hipError_t launch_batch_generate(const BatchSystem* sys, int nsys, uint64_t seed, float* A,
                                  double* b64, hipStream_t s);
+// pre_bad non-null: the systems were factorised by launch_batch_step (their
+// zero-pivot flags in pre_bad); only the substitution and refinement run, and
+// maxm only sizes their LDS vectors (npad doubles and npad floats)
 hipError_t launch_batch_solve(int threads, bool f16, int grid, const BatchSystem* sys,
                               const int* order, int nsys, int* counter, float* A, const float* A0,
                               const double* b64, double* x64, float* xs, double* rs, int max_iter,
-                              double tol_per_n, BatchResult* out, int maxm, hipStream_t s);
+                              double tol_per_n, BatchResult* out, int maxm, hipStream_t s,
+                              const int* pre_bad = nullptr);
+// Step k0 of the blocked factorisation over the first act_diag systems of
+// order (all with npad > k0; the first act of them have npad > k0 + 32 and a
+// trailing size of at most mmax): diagonal blocks (Dg: 32 x 33 floats per
+// system; bad[s]: zero or non-finite pivot, set at k0 = 0, raised after),
+// strips (Lg, Tg: sstride >= mmax * 32 elements per system, FP16 or FP32 as
+// f16) and trailing updates.
+hipError_t launch_batch_step(bool f16, const BatchSystem* sys, const int* order, int k0,
+                             int act_diag, int act, int mmax, float* A, float* Dg, int* bad,
+                             void* Lg, void* Tg, size_t sstride, hipStream_t s);

@@ -142,6 +142,7 @@ def load_library():
         "hpl_batch_generate": ([P, u64], i32),
         "hpl_batch_set_system": ([P, i32, P, u64, P], i32),
         "hpl_batch_autotune": ([P, i32, i32, dbl, P], i32),
+        "hpl_batch_set_blocked": ([P, i32], i32),
         "hpl_batch_solve": ([P, i32, i32, dbl, C.POINTER(_BatchStats)], i32),
         "hpl_batch_get": ([P, i32, P, C.POINTER(i32), C.POINTER(dbl)], i32),
         "hpl_batch_get_system": ([P, i32, P, P], i32),
@@ -498,6 +499,11 @@ class Batch:
         self._check(self._L.hpl_batch_autotune(self._b, PRECISIONS[prec], max_iter, tol_per_n,
                                                th.ctypes.data))
         return th
+
+    def set_blocked(self, blocked: bool):
+        """Factorise every bucket blocked (one launch per 32-wide step over
+        the bucket) or one workgroup per system, until the next autotune."""
+        self._check(self._L.hpl_batch_set_blocked(self._b, int(bool(blocked))))
 
     def solve(self, prec: str = "fp16", max_iter: int = 20, tol_per_n: float = 1e-14) -> dict:
         st = _BatchStats()

@@ -255,10 +255,17 @@ hpl_status hpl_batch_generate(hpl_batch* batch, uint64_t seed);
 /* Host system s: row-major a (lda >= n, M = a^T), FP64 right-hand side. */
 hpl_status hpl_batch_set_system(hpl_batch* batch, int s, const float* a, uint64_t lda,
                                 const double* b);
-/* Time each candidate workgroup size per size bucket, keep the fastest;
- * threads[4] (may be NULL) receives the choice per bucket (0: empty). */
+/* Time each candidate per size bucket -- workgroup sizes 128, 256, 512, each
+ * with the persistent factorisation (one workgroup per system) and with the
+ * blocked one (one launch per 32-wide step over the bucket's systems) --
+ * and keep the fastest; threads[4] (may be NULL) receives the choice per
+ * bucket: the workgroup size, negated for the blocked factorisation (0:
+ * empty bucket).  No counterpart in the reference (one system per run). */
 hpl_status hpl_batch_autotune(hpl_batch* batch, hpl_prec prec, int max_iter, double tol_per_n,
                               int* threads);
+/* Factorise every bucket blocked (1) or one workgroup per system (0) until
+ * the next autotune; both give bit-identical factors. */
+hpl_status hpl_batch_set_blocked(hpl_batch* batch, int blocked);
 hpl_status hpl_batch_solve(hpl_batch* batch, hpl_prec prec, int max_iter, double tol_per_n,
                            hpl_batch_stats* stats);
 /* Refined solution (n doubles), corrections and backward error of system s. */

@@ -101,7 +101,7 @@ def batch_orders(count, lo, hi, seed):
 
 
 @pytest.mark.parametrize("TPW", [4, 8])
-@pytest.mark.parametrize("NT", [128, 256])
+@pytest.mark.parametrize("NT", [128, 256, 512])
 def test_every_access_in_bounds(TPW, NT):
     workloads = [
         np.array([33, 32, 100, 257, 500, 64, 1, 5, 992, 130]),  # test_gpu_batch mixed orders

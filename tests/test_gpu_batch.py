@@ -3,8 +3,9 @@
 Bars: the device generator is bit-identical to the oracle's; every system's
 refined solution reaches backward error <= 1e-14 * n and matches the FP64
 oracle (gefa/gesl restated in FP64) to 1e-10 relative; each system's result
-is bit-identical whether it is solved alone or in the mix, and across graph
-replays with other device work in between.
+is bit-identical whether it is solved alone or in the mix, across graph
+replays with other device work in between, and whether the factorisation runs
+one workgroup per system or blocked (one launch per step over the bucket).
 """
 import numpy as np
 import pytest
@@ -77,9 +78,37 @@ def test_autotune_and_fp32(ctx):
     bt = Batch(ctx, orders)
     bt.generate(3)
     th = bt.autotune("fp16")
-    assert all(t in (0, 128, 256) for t in th)
+    # the workgroup size, negated for the blocked factorisation
+    assert all(t in (0, 128, 256, 512, -128, -256, -512) for t in th)
     st = bt.solve("fp16")
     assert st["failed"] == 0
     st32 = bt.solve("fp32")
     assert st32["failed"] == 0 and st32["max_iterations"] <= 3
+    bt.close()
+
+
+@pytest.mark.parametrize("prec,hi", [("fp16", 1024), ("fp32", 480)])
+def test_blocked_equals_persistent(ctx, prec, hi):
+    """The blocked factorisation applies the persistent kernel's operations in
+    the same order: every system's refined solution, iteration count and
+    backward error are bit-identical between the two modes."""
+    orders = np.concatenate([O.batch_orders(120, 32, hi, 5), [hi, 33, 1, 64]]).astype(np.int32)
+    print(f"batch blocked vs persistent {prec} n<={hi}", flush=True)
+    bt = Batch(ctx, orders)
+    bt.generate(5)
+    bt.set_blocked(False)
+    st0 = bt.solve(prec)
+    ref = [bt.get(s) for s in range(len(orders))]
+    bt.set_blocked(True)
+    st1 = bt.solve(prec)
+    assert st0["failed"] == 0 and st1["failed"] == 0
+    assert st1["zero_pivots"] == st0["zero_pivots"] == 0
+    for s in range(len(orders)):
+        x, it, be = bt.get(s)
+        assert np.array_equal(x, ref[s][0]), s
+        assert it == ref[s][1] and be == ref[s][2]
+    # and again after a replay of the blocked graph
+    bt.solve(prec)
+    for s in (0, len(orders) - 1):
+        assert np.array_equal(bt.get(s)[0], ref[s][0])
     bt.close()
