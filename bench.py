@@ -536,6 +536,9 @@ def main():
     ap.add_argument("--only-fp32", action="store_true",
                     help="run the plain FP32 section alone and print it as the JSON line "
                          "(for profiling that section)")
+    ap.add_argument("--only-batched", action="store_true",
+                    help="run only the batched section (autotune, then five graph replays) and "
+                         "print its JSON (for profiling)")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the pivoted, FP64, mixed-precision sweep, batched and serving "
                          "sections")
@@ -561,6 +564,11 @@ def main():
         sec = fp32_section(ctx, args.fp32_n, B, args.fp32_steps, args.warmup)
         ctx.close()
         print(json.dumps({"fp32_plain": sec, "device": dev}), flush=True)
+        return
+    if args.only_batched:
+        sec = batched_section(ctx)
+        ctx.close()
+        print(json.dumps({"batched": sec, "device": dev}), flush=True)
         return
     h = headline(ctx, n, B, args.d, args.steps, args.warmup)
     tot = h["tot"]

@@ -49,9 +49,31 @@ __device__ __forceinline__ double block_max(double v, double* red) {
     return r;
 }
 
+// a[q] = sum_p A[p][q] x[p] over the 32 rows p of a block, for 4 adjacent
+// columns (16-byte loads, 8 rows in flight; each column's sum in the order
+// p = 0..31)
+__device__ __forceinline__ void gemv4(const float* __restrict__ A, int npad,
+                                      const float* __restrict__ x, f32x4& a) {
+#pragma unroll
+    for (int h = 0; h < 32; h += 8) {
+        f32x4 v[8];
+#pragma unroll
+        for (int p = 0; p < 8; ++p)
+            v[p] = *reinterpret_cast<const f32x4*>(A + (int64_t)(h + p) * npad);
+#pragma unroll
+        for (int p = 0; p < 8; ++p) {
+            const float xp = x[h + p];
+            a[0] = fmaf(v[p][0], xp, a[0]);
+            a[1] = fmaf(v[p][1], xp, a[1]);
+            a[2] = fmaf(v[p][2], xp, a[2]);
+            a[3] = fmaf(v[p][3], xp, a[3]);
+        }
+    }
+}
+
 // Solve M x = rhs in place on x (FP32, in LDS), factor in A (npad, dgefa
-// encoding).  The diagonal blocks' solves and the GEMV loops keep all 32
-// factor loads of a block in flight.
+// encoding).  The diagonal blocks' solves keep all 32 factor loads of a block
+// in flight; the GEMV loops read 4 adjacent columns per thread.
 template <int NT>
 __device__ __forceinline__ void substitute(const float* __restrict__ A, int npad,
                                            float* __restrict__ x) {
@@ -73,38 +95,40 @@ __device__ __forceinline__ void substitute(const float* __restrict__ A, int npad
             x[k0 + lane] = xi;
         }
         __syncthreads();
-        for (int e = k0 + 32 + t; e < npad; e += NT) {
-            float a = 0.0f;
-#pragma unroll
-            for (int p = 0; p < 32; ++p) a = fmaf(A[(int64_t)(k0 + p) * npad + e], x[k0 + p], a);
-            x[e] += a;
+        for (int e = k0 + 32 + 4 * t; e < npad; e += 4 * NT) {
+            f32x4 a = {0.0f, 0.0f, 0.0f, 0.0f};
+            gemv4(A + (int64_t)k0 * npad + e, npad, x + k0, a);
+            x[e] += a[0];
+            x[e + 1] += a[1];
+            x[e + 2] += a[2];
+            x[e + 3] += a[3];
         }
         __syncthreads();
     }
     // backward: x_k = x_k / u_kk (stored -1/u_kk), x_i -= x_k u_ik (i < k)
     for (int k0 = npad - 32; k0 >= 0; k0 -= 32) {
         if (wave == 0 && lane < 32) {
-            float d[32], dg[32];
+            float d[32];
 #pragma unroll
-            for (int p = 0; p < 32; ++p) {
-                d[p] = A[(int64_t)(k0 + p) * npad + k0 + lane];
-                dg[p] = A[(int64_t)(k0 + p) * npad + k0 + p];
-            }
+            for (int p = 0; p < 32; ++p) d[p] = A[(int64_t)(k0 + p) * npad + k0 + lane];
             float xi = x[k0 + lane];
 #pragma unroll
             for (int p = 31; p >= 0; --p) {
-                float sc = __shfl(xi, p) * dg[p];
+                // the diagonal element (k0+p, k0+p) is lane p's d[p]
+                float sc = __shfl(xi, p) * __shfl(d[p], p);
                 if (lane == p) xi = -sc;
                 if (lane < p) xi += sc * d[p];
             }
             x[k0 + lane] = xi;
         }
         __syncthreads();
-        for (int e = t; e < k0; e += NT) {
-            float a = 0.0f;
-#pragma unroll
-            for (int p = 0; p < 32; ++p) a = fmaf(A[(int64_t)(k0 + p) * npad + e], x[k0 + p], a);
-            x[e] -= a;
+        for (int e = 4 * t; e < k0; e += 4 * NT) {
+            f32x4 a = {0.0f, 0.0f, 0.0f, 0.0f};
+            gemv4(A + (int64_t)k0 * npad + e, npad, x + k0, a);
+            x[e] -= a[0];
+            x[e + 1] -= a[1];
+            x[e + 2] -= a[2];
+            x[e + 3] -= a[3];
         }
         __syncthreads();
     }
@@ -112,8 +136,13 @@ __device__ __forceinline__ void substitute(const float* __restrict__ A, int npad
 
 // LDS layout (dynamic): D[32][33] f32 (the refinement's reduction scratch
 // aliases it) | Left[MAXM][40] T | TopT[MAXM][40] T
-template <int NT, bool F16>
-__global__ __launch_bounds__(NT) void batch_solve_kernel(
+// FACTOR false: the systems were factorised by the blocked kernels below
+// (their zero-pivot flags in pre_bad): substitution and refinement only, in a
+// kernel sized for three waves per SIMD (its registers no longer hold the
+// factorisation's tiles), so several systems share a CU.
+template <int NT, bool F16, bool FACTOR>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(FACTOR ? 1 : 3))) void
+batch_solve_kernel(
     const BatchSystem* __restrict__ sys, const int* __restrict__ order, int nsys,
     int* __restrict__ counter, float* __restrict__ A, const float* __restrict__ A0,
     const double* __restrict__ b64, double* __restrict__ x64, float* __restrict__ xs,
@@ -143,10 +172,8 @@ __global__ __launch_bounds__(NT) void batch_solve_kernel(
         bool bad = false;
 
         // ------------------------------------------------------------ factor
-        // (pre_bad: factorised by the blocked kernels below, which flagged
-        // zero or non-finite pivots there)
-        if (pre_bad) bad = pre_bad[s] != 0;
-        for (int k0 = 0; k0 < (pre_bad ? 0 : npad); k0 += 32) {
+        if constexpr (!FACTOR) bad = pre_bad[s] != 0;
+        for (int k0 = 0; k0 < (FACTOR ? npad : 0); k0 += 32) {
             if (wave == 0) {
                 // diagonal block: lane c < 32 holds column k0+c, rows 0..31
                 float d[32];
@@ -284,34 +311,44 @@ __global__ __launch_bounds__(NT) void batch_solve_kernel(
         __syncthreads();
         double anorm = 0.0, berr = 0.0;
         // x is staged in LDS (the strip copies are dead after the
-        // factorisation) so the residual's inner loop streams only the matrix,
-        // 32 rows per unrolled chunk with their loads in flight together; the
-        // sums still run over j in increasing order
+        // factorisation) so the residual's inner loop streams only the matrix:
+        // each thread takes 4 adjacent rows i (16-byte loads of the stored
+        // transpose), 8 j at a time in flight; every sum still runs over j in
+        // increasing order
         double* xsh = reinterpret_cast<double*>(Left);
         int it = 0;
         for (;; ++it) {
             double rmax = 0.0, amax = 0.0, xmax = 0.0, bmax = 0.0;
             for (int i = t; i < npad; i += NT) xsh[i] = x[i];
             __syncthreads();
-            for (int i = t; i < npad; i += NT) {
-                double acc = 0.0, sa = 0.0;
-                for (int j0 = 0; j0 < npad; j0 += 32) {
-                    float v32[32];
+            for (int i0 = 4 * t; i0 < npad; i0 += 4 * NT) {
+                double acc[4] = {0.0, 0.0, 0.0, 0.0}, sa[4] = {0.0, 0.0, 0.0, 0.0};
+                for (int j0 = 0; j0 < npad; j0 += 8) {
+                    f32x4 v4[8];
 #pragma unroll
-                    for (int jj = 0; jj < 32; ++jj) v32[jj] = a0[(int64_t)(j0 + jj) * npad + i];
+                    for (int jj = 0; jj < 8; ++jj)
+                        v4[jj] = *reinterpret_cast<const f32x4*>(a0 + (int64_t)(j0 + jj) * npad + i0);
 #pragma unroll
-                    for (int jj = 0; jj < 32; ++jj) {
-                        const double v = (double)v32[jj];
-                        acc = fma(v, xsh[j0 + jj], acc);
-                        sa += fabs(v);
+                    for (int jj = 0; jj < 8; ++jj) {
+                        const double xj = xsh[j0 + jj];
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            const double v = (double)v4[jj][q];
+                            acc[q] = fma(v, xj, acc[q]);
+                            sa[q] += fabs(v);
+                        }
                     }
                 }
-                const double ri = b[i] - acc;
-                r[i] = ri;
-                rmax = fmax(rmax, fabs(ri));
-                amax = fmax(amax, sa);
-                xmax = fmax(xmax, fabs(x[i]));
-                bmax = fmax(bmax, fabs(b[i]));
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int i = i0 + q;
+                    const double ri = b[i] - acc[q];
+                    r[i] = ri;
+                    rmax = fmax(rmax, fabs(ri));
+                    amax = fmax(amax, sa[q]);
+                    xmax = fmax(xmax, fabs(x[i]));
+                    bmax = fmax(bmax, fabs(b[i]));
+                }
             }
             rmax = block_max<NT>(rmax, red);
             if (it == 0) anorm = block_max<NT>(amax, red);
@@ -524,12 +561,18 @@ __global__ void reset_counter_kernel(int* c) {
 
 hipError_t batch_prepare() {
     const int lds = (int)kBatchLdsLimit;
-    const void* ks[] = {reinterpret_cast<const void*>(batch_solve_kernel<128, true>),
-                        reinterpret_cast<const void*>(batch_solve_kernel<128, false>),
-                        reinterpret_cast<const void*>(batch_solve_kernel<256, true>),
-                        reinterpret_cast<const void*>(batch_solve_kernel<256, false>),
-                        reinterpret_cast<const void*>(batch_solve_kernel<512, true>),
-                        reinterpret_cast<const void*>(batch_solve_kernel<512, false>)};
+    const void* ks[] = {reinterpret_cast<const void*>(batch_solve_kernel<128, true, true>),
+                        reinterpret_cast<const void*>(batch_solve_kernel<128, false, true>),
+                        reinterpret_cast<const void*>(batch_solve_kernel<256, true, true>),
+                        reinterpret_cast<const void*>(batch_solve_kernel<256, false, true>),
+                        reinterpret_cast<const void*>(batch_solve_kernel<512, true, true>),
+                        reinterpret_cast<const void*>(batch_solve_kernel<512, false, true>),
+                        reinterpret_cast<const void*>(batch_solve_kernel<128, true, false>),
+                        reinterpret_cast<const void*>(batch_solve_kernel<128, false, false>),
+                        reinterpret_cast<const void*>(batch_solve_kernel<256, true, false>),
+                        reinterpret_cast<const void*>(batch_solve_kernel<256, false, false>),
+                        reinterpret_cast<const void*>(batch_solve_kernel<512, true, false>),
+                        reinterpret_cast<const void*>(batch_solve_kernel<512, false, false>)};
     for (const void* k : ks) {
         hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
         if (e != hipSuccess) return e;
@@ -577,10 +620,17 @@ hipError_t launch_batch_solve(int threads, bool f16, int grid, const BatchSystem
     const size_t lds = batch_lds_bytes(maxm, f16);
     if (lds > kBatchLdsLimit) return hipErrorInvalidValue;
     reset_counter_kernel<<<1, 64, 0, s>>>(counter);
-#define HPL_BATCH(NTV, F)                                                                     \
-    batch_solve_kernel<NTV, F><<<grid, NTV, lds, s>>>(sys, order, nsys, counter, A, A0, b64, x64, \
-                                                     xs, rs, max_iter, tol_per_n, out, maxm,  \
-                                                     pre_bad)
+#define HPL_BATCH(NTV, F)                                                                   \
+    do {                                                                                    \
+        if (pre_bad)                                                                        \
+            batch_solve_kernel<NTV, F, false><<<grid, NTV, lds, s>>>(                       \
+                sys, order, nsys, counter, A, A0, b64, x64, xs, rs, max_iter, tol_per_n, out, \
+                maxm, pre_bad);                                                             \
+        else                                                                                \
+            batch_solve_kernel<NTV, F, true><<<grid, NTV, lds, s>>>(                        \
+                sys, order, nsys, counter, A, A0, b64, x64, xs, rs, max_iter, tol_per_n, out, \
+                maxm, nullptr);                                                             \
+    } while (0)
     if (threads == 512) {
         if (f16) HPL_BATCH(512, true); else HPL_BATCH(512, false);
     } else if (threads == 256) {
