@@ -34,10 +34,13 @@ struct Bucket {
     int threads = 256;  // workgroup size (autotuned)
     bool blocked = false;  // blocked factorisation (autotuned)
     // scratch of the blocked factorisation: diagonal blocks (count x 1056
-    // floats), strip copies (count x maxm x 32 floats each)
+    // floats), strip copies of two consecutive steps (count x maxm x 32
+    // floats each)
     float* dDg = nullptr;
     void* dLg = nullptr;
     void* dTg = nullptr;
+    void* dLg2 = nullptr;
+    void* dTg2 = nullptr;
 };
 
 }  // namespace
@@ -116,6 +119,8 @@ hpl_status ensure_blocked(hpl_batch* b, Bucket& k) {
     BCHK(b, hipMalloc(&k.dDg, (size_t)k.count * 1056 * sizeof(float)));
     BCHK(b, hipMalloc(&k.dLg, strip));
     BCHK(b, hipMalloc(&k.dTg, strip));
+    BCHK(b, hipMalloc(&k.dLg2, strip));
+    BCHK(b, hipMalloc(&k.dTg2, strip));
     return HPL_SUCCESS;
 }
 
@@ -131,18 +136,22 @@ hpl_status enqueue_bucket(hpl_batch* b, int i, bool f16, int max_iter, double to
         return HPL_SUCCESS;
     }
     if (!k.dDg) return bfail(b, HPL_ERR_STATE, "blocked scratch of bucket %d missing", i);
-    // steps over the systems still longer than the step: a prefix of the
-    // bucket's largest-first order
+    // pairs of steps over the systems still longer than the step: a prefix
+    // of the bucket's largest-first order
     const int* order = b->order.data() + k.begin;
     const int top = b->sys[order[0]].npad;
     const size_t sstride = (size_t)std::max(k.maxm, 32) * 32;
-    for (int k0 = 0; k0 < top; k0 += 32) {
-        int act_diag = 0, act = 0;
-        while (act_diag < k.count && b->sys[order[act_diag]].npad > k0) ++act_diag;
-        while (act < k.count && b->sys[order[act]].npad > k0 + 32) ++act;
+    auto active = [&](int above) {
+        int a = 0;
+        while (a < k.count && b->sys[order[a]].npad > above) ++a;
+        return a;
+    };
+    for (int k0 = 0; k0 < top; k0 += 64) {
+        const int act_diag = active(k0), act = active(k0 + 32), act2 = active(k0 + 64);
         const int mmax = act ? top - k0 - 32 : 0;
-        BCHK(b, launch_batch_step(f16, b->dSys, b->dOrder + k.begin, k0, act_diag, act, mmax,
-                                  b->dA, k.dDg, b->dBad, k.dLg, k.dTg, sstride, s));
+        BCHK(b, launch_batch_step2(f16, b->dSys, b->dOrder + k.begin, k0, act_diag, act, act2,
+                                   mmax, b->dA, k.dDg, b->dBad, k.dLg, k.dTg, k.dLg2, k.dTg2,
+                                   sstride, s));
     }
     // substitutions and refinement: LDS for npad doubles and npad floats
     Bucket solve = k;
@@ -265,7 +274,7 @@ void hpl_batch_destroy(hpl_batch* b) {
                     (void*)b->dRes, (void*)b->dBad})
         if (p) (void)hipFree(p);
     for (auto& k : b->buckets)
-        for (void* p : {(void*)k.dDg, k.dLg, k.dTg})
+        for (void* p : {(void*)k.dDg, k.dLg, k.dTg, k.dLg2, k.dTg2})
             if (p) (void)hipFree(p);
     for (int i = 0; i < kBuckets; ++i) {
         if (b->sb[i]) (void)hipStreamDestroy(b->sb[i]);
@@ -364,6 +373,36 @@ hpl_status hpl_batch_autotune(hpl_batch* b, hpl_prec prec, int max_iter, double 
         k.threads = best_t;
         k.blocked = best_blk;
     }
+    // The buckets run concurrently: a bucket timed alone can prefer one
+    // workgroup per system where, beside the others, the blocked mode (which
+    // spreads each step over the device) finishes first.  Time the whole
+    // solve with the per-bucket choice and with every bucket blocked (same
+    // workgroup sizes), and keep the faster.
+    auto time_all = [&](float* out_ms) -> hpl_status {
+        float sum = 0.f;
+        for (int rep = 0; rep < 3; ++rep) {
+            BCHK(b, hipEventRecord(b->t0, b->s0));
+            hpl_status st = enqueue_solve(b, f16, max_iter, tol_per_n);
+            if (st) return st;
+            BCHK(b, hipEventRecord(b->t1, b->s0));
+            BCHK(b, hipEventSynchronize(b->t1));
+            float ms = 0.f;
+            BCHK(b, hipEventElapsedTime(&ms, b->t0, b->t1));
+            if (rep) sum += ms;  // first run warms up
+        }
+        *out_ms = sum;
+        return HPL_SUCCESS;
+    };
+    bool chosen[kBuckets];
+    for (int i = 0; i < kBuckets; ++i) chosen[i] = b->buckets[i].blocked;
+    float t_chosen = 0.f, t_blocked = 0.f;
+    hpl_status st1 = time_all(&t_chosen);
+    if (st1) return st1;
+    for (auto& k : b->buckets) k.blocked = k.count > 0;
+    st1 = time_all(&t_blocked);
+    if (st1) return st1;
+    if (t_chosen <= t_blocked)
+        for (int i = 0; i < kBuckets; ++i) b->buckets[i].blocked = chosen[i];
     if (threads_out)
         for (int i = 0; i < kBuckets; ++i) {
             const Bucket& k = b->buckets[i];

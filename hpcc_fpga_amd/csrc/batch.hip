@@ -480,29 +480,14 @@ __global__ __launch_bounds__(256) void bstep_strips_kernel(const BatchSystem* __
     }
 }
 
-// the trailing update of step k0: one 32 x 32 tile per wave
+// acc += the step's rank-32 product on tile (ti, tj) of its trailing matrix
+// (strip rows ti*32.. of Left, tj*32.. of TopT)
 template <bool F16>
-__global__ __launch_bounds__(256) void bstep_update_kernel(const BatchSystem* __restrict__ sys,
-                                                           const int* __restrict__ order, int k0,
-                                                           float* __restrict__ A,
-                                                           const void* __restrict__ Lv,
-                                                           const void* __restrict__ Tv,
-                                                           size_t sstride) {
+__device__ __forceinline__ void bstep_tile_mfma(f32x16& acc, const void* Lv, const void* Tv,
+                                                int ti, int tj, int lr, int lh) {
     typedef typename std::conditional<F16, _Float16, float>::type T;
-    const int j = blockIdx.y, s = order[j];
-    const int npad = sys[s].npad, r0 = k0 + 32, m = npad - r0;
-    const int tiles = m / 32, ntile = tiles * tiles;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int tt = blockIdx.x * 4 + wave;
-    if (tt >= ntile) return;  // wave-uniform; no barrier in this kernel
-    const int lr = lane & 31, lh = lane >> 5;
-    const int ti = tt / tiles, tj = tt % tiles;
-    const T* Left = static_cast<const T*>(Lv) + (size_t)j * sstride;
-    const T* TopT = static_cast<const T*>(Tv) + (size_t)j * sstride;
-    float* cbase = A + sys[s].a_off + (int64_t)(r0 + ti * 32) * npad + r0 + tj * 32;
-    f32x16 acc;
-#pragma unroll
-    for (int rr = 0; rr < 16; ++rr) acc[rr] = cbase[(int64_t)acc_row(rr, lane) * npad + lr];
+    const T* Left = static_cast<const T*>(Lv);
+    const T* TopT = static_cast<const T*>(Tv);
     if constexpr (F16) {
 #pragma unroll
         for (int ks = 0; ks < 2; ++ks) {
@@ -520,6 +505,51 @@ __global__ __launch_bounds__(256) void bstep_update_kernel(const BatchSystem* __
             acc = mfma32(av, bv, acc);
         }
     }
+}
+
+// Trailing update of step k0, one 32 x 32 tile per wave.  MODE 0: every tile
+// of the step's trailing matrix.  MODE 1: its first tile row and column only
+// (the next step's diagonal block and strips).  MODE 2: the other tiles, with
+// step k0's update and then step k0+32's (strips Lv2 / Tv2, tile indices one
+// less): the same products in the same order as two MODE 0 passes, since the
+// FP32 accumulator is what a pass would store and reload.
+template <bool F16, int MODE>
+__global__ __launch_bounds__(256) void bstep_update_kernel(
+    const BatchSystem* __restrict__ sys, const int* __restrict__ order, int k0,
+    float* __restrict__ A, const void* __restrict__ Lv, const void* __restrict__ Tv,
+    const void* __restrict__ Lv2, const void* __restrict__ Tv2, size_t sstride) {
+    typedef typename std::conditional<F16, _Float16, float>::type T;
+    const int j = blockIdx.y, s = order[j];
+    const int npad = sys[s].npad, r0 = k0 + 32, m = npad - r0;
+    const int tiles = m / 32;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int tt = blockIdx.x * 4 + wave;
+    int ti, tj;
+    if constexpr (MODE == 0) {
+        if (tt >= tiles * tiles) return;  // wave-uniform; no barrier in this kernel
+        ti = tt / tiles;
+        tj = tt % tiles;
+    } else if constexpr (MODE == 1) {
+        if (tt >= 2 * tiles - 1) return;
+        ti = tt < tiles ? 0 : tt - tiles + 1;
+        tj = tt < tiles ? tt : 0;
+    } else {
+        const int t2 = tiles - 1;
+        if (tt >= t2 * t2) return;
+        ti = 1 + tt / t2;
+        tj = 1 + tt % t2;
+    }
+    const int lr = lane & 31, lh = lane >> 5;
+    float* cbase = A + sys[s].a_off + (int64_t)(r0 + ti * 32) * npad + r0 + tj * 32;
+    f32x16 acc;
+#pragma unroll
+    for (int rr = 0; rr < 16; ++rr) acc[rr] = cbase[(int64_t)acc_row(rr, lane) * npad + lr];
+    bstep_tile_mfma<F16>(acc, static_cast<const T*>(Lv) + (size_t)j * sstride,
+                         static_cast<const T*>(Tv) + (size_t)j * sstride, ti, tj, lr, lh);
+    if constexpr (MODE == 2)
+        bstep_tile_mfma<F16>(acc, static_cast<const T*>(Lv2) + (size_t)j * sstride,
+                             static_cast<const T*>(Tv2) + (size_t)j * sstride, ti - 1, tj - 1, lr,
+                             lh);
 #pragma unroll
     for (int rr = 0; rr < 16; ++rr) cbase[(int64_t)acc_row(rr, lane) * npad + lr] = acc[rr];
 }
@@ -603,10 +633,53 @@ hipError_t launch_batch_step(bool f16, const BatchSystem* sys, const int* order,
         const dim3 gu((unsigned)((tiles * tiles + 3) / 4), (unsigned)act);
         if (f16) {
             bstep_strips_kernel<true><<<gs, 256, 0, s>>>(sys, order, k0, A, Dg, Lg, Tg, sstride);
-            bstep_update_kernel<true><<<gu, 256, 0, s>>>(sys, order, k0, A, Lg, Tg, sstride);
+            bstep_update_kernel<true, 0><<<gu, 256, 0, s>>>(sys, order, k0, A, Lg, Tg, nullptr,
+                                                            nullptr, sstride);
         } else {
             bstep_strips_kernel<false><<<gs, 256, 0, s>>>(sys, order, k0, A, Dg, Lg, Tg, sstride);
-            bstep_update_kernel<false><<<gu, 256, 0, s>>>(sys, order, k0, A, Lg, Tg, sstride);
+            bstep_update_kernel<false, 0><<<gu, 256, 0, s>>>(sys, order, k0, A, Lg, Tg, nullptr,
+                                                             nullptr, sstride);
+        }
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_step2(bool f16, const BatchSystem* sys, const int* order, int k0,
+                              int act_diag, int act, int act2, int mmax, float* A, float* Dg,
+                              int* bad, void* Lg, void* Tg, void* Lg2, void* Tg2, size_t sstride,
+                              hipStream_t s) {
+    if (act_diag < 1 || act > act_diag || act2 > act || act > 65535) return hipErrorInvalidValue;
+    if (act > 0 && (mmax < 32 || mmax % 32 || (size_t)mmax * 32 > sstride))
+        return hipErrorInvalidValue;
+    const int tiles = mmax / 32;
+    bstep_diag_kernel<<<act_diag, 64, 0, s>>>(sys, order, k0, A, Dg, bad);
+    if (act == 0) return hipGetLastError();
+    const dim3 gs((unsigned)((mmax + 255) / 256), (unsigned)act);
+    const dim3 gn((unsigned)((2 * tiles - 1 + 3) / 4), (unsigned)act);
+    if (f16) {
+        bstep_strips_kernel<true><<<gs, 256, 0, s>>>(sys, order, k0, A, Dg, Lg, Tg, sstride);
+        bstep_update_kernel<true, 1><<<gn, 256, 0, s>>>(sys, order, k0, A, Lg, Tg, nullptr,
+                                                        nullptr, sstride);
+    } else {
+        bstep_strips_kernel<false><<<gs, 256, 0, s>>>(sys, order, k0, A, Dg, Lg, Tg, sstride);
+        bstep_update_kernel<false, 1><<<gn, 256, 0, s>>>(sys, order, k0, A, Lg, Tg, nullptr,
+                                                         nullptr, sstride);
+    }
+    // step k0 + 32 on the systems longer than it
+    bstep_diag_kernel<<<act, 64, 0, s>>>(sys, order, k0 + 32, A, Dg, bad);
+    if (act2 > 0 && tiles > 1) {
+        const dim3 gs2((unsigned)((mmax - 32 + 255) / 256), (unsigned)act2);
+        const dim3 gw((unsigned)(((tiles - 1) * (tiles - 1) + 3) / 4), (unsigned)act2);
+        if (f16) {
+            bstep_strips_kernel<true><<<gs2, 256, 0, s>>>(sys, order, k0 + 32, A, Dg, Lg2, Tg2,
+                                                          sstride);
+            bstep_update_kernel<true, 2><<<gw, 256, 0, s>>>(sys, order, k0, A, Lg, Tg, Lg2, Tg2,
+                                                            sstride);
+        } else {
+            bstep_strips_kernel<false><<<gs2, 256, 0, s>>>(sys, order, k0 + 32, A, Dg, Lg2, Tg2,
+                                                           sstride);
+            bstep_update_kernel<false, 2><<<gw, 256, 0, s>>>(sys, order, k0, A, Lg, Tg, Lg2, Tg2,
+                                                             sstride);
         }
     }
     return hipGetLastError();

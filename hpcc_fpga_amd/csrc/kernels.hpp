@@ -262,3 +262,13 @@ hipError_t launch_batch_solve(int threads, bool f16, int grid, const BatchSystem
 hipError_t launch_batch_step(bool f16, const BatchSystem* sys, const int* order, int k0,
                              int act_diag, int act, int mmax, float* A, float* Dg, int* bad,
                              void* Lg, void* Tg, size_t sstride, hipStream_t s);
+// Steps k0 and k0 + 32 together: act_diag systems with npad > k0, act with
+// npad > k0 + 32 (trailing size <= mmax after step k0), act2 with npad > k0 +
+// 64.  Step k0 updates only the next step's diagonal block and strips; the
+// other tiles get both steps' updates in one pass (the trailing matrix is
+// read and written once for two steps), bit-identical to two single steps.
+// Step k0's strips in Lg / Tg, step k0 + 32's in Lg2 / Tg2.
+hipError_t launch_batch_step2(bool f16, const BatchSystem* sys, const int* order, int k0,
+                              int act_diag, int act, int act2, int mmax, float* A, float* Dg,
+                              int* bad, void* Lg, void* Tg, void* Lg2, void* Tg2, size_t sstride,
+                              hipStream_t s);

@@ -258,9 +258,11 @@ hpl_status hpl_batch_set_system(hpl_batch* batch, int s, const float* a, uint64_
 /* Time each candidate per size bucket -- workgroup sizes 128, 256, 512, each
  * with the persistent factorisation (one workgroup per system) and with the
  * blocked one (one launch per 32-wide step over the bucket's systems) --
- * and keep the fastest; threads[4] (may be NULL) receives the choice per
- * bucket: the workgroup size, negated for the blocked factorisation (0:
- * empty bucket).  No counterpart in the reference (one system per run). */
+ * and keep the fastest; then time the whole solve with that choice and with
+ * every bucket blocked, and keep the faster.  threads[4] (may be NULL)
+ * receives the choice per bucket: the workgroup size, negated for the
+ * blocked factorisation (0: empty bucket).  No counterpart in the reference
+ * (one system per run). */
 hpl_status hpl_batch_autotune(hpl_batch* batch, hpl_prec prec, int max_iter, double tol_per_n,
                               int* threads);
 /* Factorise every bucket blocked (1) or one workgroup per system (0) until
