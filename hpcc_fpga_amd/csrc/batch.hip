@@ -432,15 +432,15 @@ __global__ __launch_bounds__(256) void bstep_strips_kernel(const BatchSystem* __
     typedef typename std::conditional<F16, _Float16, float>::type T;
     const int j = blockIdx.y, s = order[j];
     const int npad = sys[s].npad, r0 = k0 + 32, m = npad - r0;
-    if ((int)blockIdx.x * 256 >= m) return;  // the whole workgroup, before any barrier
-    __shared__ float D[32 * 33];
-    for (int e = threadIdx.x; e < 32 * 33; e += 256) D[e] = Dg[(size_t)j * 1056 + e];
-    __syncthreads();
+    // the coefficients are the same for every thread: read from global at
+    // wave-uniform addresses, they arrive through scalar loads into SGPRs
+    // instead of one LDS read per multiply-add
+    const float* __restrict__ D = Dg + (size_t)j * 1056;
     float* a = A + sys[s].a_off;
     T* Left = static_cast<T*>(Lv) + (size_t)j * sstride;
     T* TopT = static_cast<T*>(Tv) + (size_t)j * sstride;
     const int c = blockIdx.x * 256 + threadIdx.x;
-    if (c >= m) return;
+    if ((int)blockIdx.x * 256 >= m || c >= m) return;
     {  // top strip: column r0 + c of rows k0..k0+31 (closed form top)
         float x[32];
 #pragma unroll
