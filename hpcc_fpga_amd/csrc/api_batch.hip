@@ -7,8 +7,9 @@
 // then the persistent kernel for the substitutions and refinement only.  The
 // autotuner picks the mode and the workgroup size by timing; the buckets'
 // launches run concurrently on forked streams and the whole solve -- restore
-// of the matrices, queue resets, the bucket launches -- is captured once into
-// a HIP graph and replayed.
+// of the matrices (skipped when every bucket is blocked: the first step then
+// reads the originals), queue resets, the bucket launches -- is captured once
+// into a HIP graph and replayed.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -124,8 +125,10 @@ hpl_status ensure_blocked(hpl_batch* b, Bucket& k) {
     return HPL_SUCCESS;
 }
 
+// from_a0: the matrices were not restored into dA; a blocked bucket's first
+// step reads the originals from dA0 instead
 hpl_status enqueue_bucket(hpl_batch* b, int i, bool f16, int max_iter, double tol,
-                          hipStream_t s) {
+                          hipStream_t s, bool from_a0 = false) {
     const Bucket& k = b->buckets[i];
     if (!k.count) return HPL_SUCCESS;
     if (!k.blocked) {
@@ -150,8 +153,8 @@ hpl_status enqueue_bucket(hpl_batch* b, int i, bool f16, int max_iter, double to
         const int act_diag = active(k0), act = active(k0 + 32), act2 = active(k0 + 64);
         const int mmax = act ? top - k0 - 32 : 0;
         BCHK(b, launch_batch_step2(f16, b->dSys, b->dOrder + k.begin, k0, act_diag, act, act2,
-                                   mmax, b->dA, k.dDg, b->dBad, k.dLg, k.dTg, k.dLg2, k.dTg2,
-                                   sstride, s));
+                                   mmax, b->dA, from_a0 ? b->dA0 : nullptr, k.dDg, b->dBad, k.dLg,
+                                   k.dTg, k.dLg2, k.dTg2, sstride, s));
     }
     // substitutions and refinement: LDS for npad doubles and npad floats
     Bucket solve = k;
@@ -163,14 +166,20 @@ hpl_status enqueue_bucket(hpl_batch* b, int i, bool f16, int max_iter, double to
     return HPL_SUCCESS;
 }
 
-// restore + all buckets, forked over the bucket streams, on s0
+// restore (unless every bucket is blocked) + all buckets, forked over the
+// bucket streams, on s0
 hpl_status enqueue_solve(hpl_batch* b, bool f16, int max_iter, double tol) {
-    BCHK(b, hipMemcpyAsync(b->dA, b->dA0, b->a_total * sizeof(float), hipMemcpyDeviceToDevice,
-                           b->s0));
+    // every element of a matrix is written by its first blocked step, which
+    // then reads the original: with every bucket blocked no restore is needed
+    bool all_blocked = true;
+    for (const auto& k : b->buckets) all_blocked = all_blocked && (k.blocked || !k.count);
+    if (!all_blocked)
+        BCHK(b, hipMemcpyAsync(b->dA, b->dA0, b->a_total * sizeof(float),
+                               hipMemcpyDeviceToDevice, b->s0));
     BCHK(b, hipEventRecord(b->fork, b->s0));
     for (int i = 0; i < kBuckets; ++i) {
         BCHK(b, hipStreamWaitEvent(b->sb[i], b->fork, 0));
-        hpl_status st = enqueue_bucket(b, i, f16, max_iter, tol, b->sb[i]);
+        hpl_status st = enqueue_bucket(b, i, f16, max_iter, tol, b->sb[i], all_blocked);
         if (st) return st;
         BCHK(b, hipEventRecord(b->join[i], b->sb[i]));
         BCHK(b, hipStreamWaitEvent(b->s0, b->join[i], 0));

@@ -386,16 +386,18 @@ batch_solve_kernel(
 __global__ __launch_bounds__(64) void bstep_diag_kernel(const BatchSystem* __restrict__ sys,
                                                         const int* __restrict__ order, int k0,
                                                         float* __restrict__ A,
+                                                        const float* __restrict__ S,
                                                         float* __restrict__ Dg,
                                                         int* __restrict__ bad_out) {
     const int j = blockIdx.x, s = order[j];
     const int npad = sys[s].npad;
     float* a = A + sys[s].a_off;
+    const float* sa = S + sys[s].a_off;
     const int lane = threadIdx.x, lr = lane & 31;
     bool bad = false;
     float d[32];
 #pragma unroll
-    for (int r = 0; r < 32; ++r) d[r] = lr == lane ? a[(int64_t)(k0 + r) * npad + k0 + lr] : 0.0f;
+    for (int r = 0; r < 32; ++r) d[r] = lr == lane ? sa[(int64_t)(k0 + r) * npad + k0 + lr] : 0.0f;
 #pragma unroll
     for (int jj = 0; jj < 32; ++jj) {
         const float piv = bcast(d[jj], jj);
@@ -425,6 +427,7 @@ template <bool F16>
 __global__ __launch_bounds__(256) void bstep_strips_kernel(const BatchSystem* __restrict__ sys,
                                                            const int* __restrict__ order, int k0,
                                                            float* __restrict__ A,
+                                                           const float* __restrict__ S,
                                                            const float* __restrict__ Dg,
                                                            void* __restrict__ Lv,
                                                            void* __restrict__ Tv,
@@ -437,6 +440,7 @@ __global__ __launch_bounds__(256) void bstep_strips_kernel(const BatchSystem* __
     // instead of one LDS read per multiply-add
     const float* __restrict__ D = Dg + (size_t)j * 1056;
     float* a = A + sys[s].a_off;
+    const float* sa = S + sys[s].a_off;
     T* Left = static_cast<T*>(Lv) + (size_t)j * sstride;
     T* TopT = static_cast<T*>(Tv) + (size_t)j * sstride;
     const int c = blockIdx.x * 256 + threadIdx.x;
@@ -444,7 +448,7 @@ __global__ __launch_bounds__(256) void bstep_strips_kernel(const BatchSystem* __
     {  // top strip: column r0 + c of rows k0..k0+31 (closed form top)
         float x[32];
 #pragma unroll
-        for (int i = 0; i < 32; ++i) x[i] = a[(int64_t)(k0 + i) * npad + r0 + c];
+        for (int i = 0; i < 32; ++i) x[i] = sa[(int64_t)(k0 + i) * npad + r0 + c];
 #pragma unroll
         for (int kk = 0; kk < 32; ++kk) {
             x[kk] = __fmul_rn(x[kk], D[kk * 33 + kk]);
@@ -460,9 +464,10 @@ __global__ __launch_bounds__(256) void bstep_strips_kernel(const BatchSystem* __
     {  // left strip: row r0 + c, columns k0..k0+31 (closed form left)
         float x[32];
         float* row = a + (int64_t)(r0 + c) * npad + k0;
+        const float* srow = sa + (int64_t)(r0 + c) * npad + k0;
 #pragma unroll
         for (int i = 0; i < 32; i += 4) {
-            const f32x4 v = *reinterpret_cast<const f32x4*>(row + i);
+            const f32x4 v = *reinterpret_cast<const f32x4*>(srow + i);
             x[i] = v[0];
             x[i + 1] = v[1];
             x[i + 2] = v[2];
@@ -516,8 +521,9 @@ __device__ __forceinline__ void bstep_tile_mfma(f32x16& acc, const void* Lv, con
 template <bool F16, int MODE>
 __global__ __launch_bounds__(256) void bstep_update_kernel(
     const BatchSystem* __restrict__ sys, const int* __restrict__ order, int k0,
-    float* __restrict__ A, const void* __restrict__ Lv, const void* __restrict__ Tv,
-    const void* __restrict__ Lv2, const void* __restrict__ Tv2, size_t sstride) {
+    float* __restrict__ A, const float* __restrict__ S, const void* __restrict__ Lv,
+    const void* __restrict__ Tv, const void* __restrict__ Lv2, const void* __restrict__ Tv2,
+    size_t sstride) {
     typedef typename std::conditional<F16, _Float16, float>::type T;
     const int j = blockIdx.y, s = order[j];
     const int npad = sys[s].npad, r0 = k0 + 32, m = npad - r0;
@@ -540,10 +546,12 @@ __global__ __launch_bounds__(256) void bstep_update_kernel(
         tj = 1 + tt % t2;
     }
     const int lr = lane & 31, lh = lane >> 5;
-    float* cbase = A + sys[s].a_off + (int64_t)(r0 + ti * 32) * npad + r0 + tj * 32;
+    const int64_t toff = sys[s].a_off + (int64_t)(r0 + ti * 32) * npad + r0 + tj * 32;
+    float* cbase = A + toff;
+    const float* sbase = S + toff;
     f32x16 acc;
 #pragma unroll
-    for (int rr = 0; rr < 16; ++rr) acc[rr] = cbase[(int64_t)acc_row(rr, lane) * npad + lr];
+    for (int rr = 0; rr < 16; ++rr) acc[rr] = sbase[(int64_t)acc_row(rr, lane) * npad + lr];
     bstep_tile_mfma<F16>(acc, static_cast<const T*>(Lv) + (size_t)j * sstride,
                          static_cast<const T*>(Tv) + (size_t)j * sstride, ti, tj, lr, lh);
     if constexpr (MODE == 2)
@@ -625,19 +633,20 @@ hipError_t launch_batch_step(bool f16, const BatchSystem* sys, const int* order,
                              int act_diag, int act, int mmax, float* A, float* Dg, int* bad,
                              void* Lg, void* Tg, size_t sstride, hipStream_t s) {
     if (act_diag < 1 || act > act_diag || act > 65535) return hipErrorInvalidValue;
-    bstep_diag_kernel<<<act_diag, 64, 0, s>>>(sys, order, k0, A, Dg, bad);
+    bstep_diag_kernel<<<act_diag, 64, 0, s>>>(sys, order, k0, A, A, Dg, bad);
     if (act > 0 && mmax > 0) {
         if (mmax % 32 || (size_t)mmax * 32 > sstride) return hipErrorInvalidValue;
         const dim3 gs((unsigned)((mmax + 255) / 256), (unsigned)act);
         const int tiles = mmax / 32;
         const dim3 gu((unsigned)((tiles * tiles + 3) / 4), (unsigned)act);
         if (f16) {
-            bstep_strips_kernel<true><<<gs, 256, 0, s>>>(sys, order, k0, A, Dg, Lg, Tg, sstride);
-            bstep_update_kernel<true, 0><<<gu, 256, 0, s>>>(sys, order, k0, A, Lg, Tg, nullptr,
+            bstep_strips_kernel<true><<<gs, 256, 0, s>>>(sys, order, k0, A, A, Dg, Lg, Tg, sstride);
+            bstep_update_kernel<true, 0><<<gu, 256, 0, s>>>(sys, order, k0, A, A, Lg, Tg, nullptr,
                                                             nullptr, sstride);
         } else {
-            bstep_strips_kernel<false><<<gs, 256, 0, s>>>(sys, order, k0, A, Dg, Lg, Tg, sstride);
-            bstep_update_kernel<false, 0><<<gu, 256, 0, s>>>(sys, order, k0, A, Lg, Tg, nullptr,
+            bstep_strips_kernel<false><<<gs, 256, 0, s>>>(sys, order, k0, A, A, Dg, Lg, Tg,
+                                                          sstride);
+            bstep_update_kernel<false, 0><<<gu, 256, 0, s>>>(sys, order, k0, A, A, Lg, Tg, nullptr,
                                                              nullptr, sstride);
         }
     }
@@ -645,41 +654,45 @@ hipError_t launch_batch_step(bool f16, const BatchSystem* sys, const int* order,
 }
 
 hipError_t launch_batch_step2(bool f16, const BatchSystem* sys, const int* order, int k0,
-                              int act_diag, int act, int act2, int mmax, float* A, float* Dg,
-                              int* bad, void* Lg, void* Tg, void* Lg2, void* Tg2, size_t sstride,
-                              hipStream_t s) {
+                              int act_diag, int act, int act2, int mmax, float* A,
+                              const float* A0, float* Dg, int* bad, void* Lg, void* Tg, void* Lg2,
+                              void* Tg2, size_t sstride, hipStream_t s) {
     if (act_diag < 1 || act > act_diag || act2 > act || act > 65535) return hipErrorInvalidValue;
     if (act > 0 && (mmax < 32 || mmax % 32 || (size_t)mmax * 32 > sstride))
         return hipErrorInvalidValue;
     const int tiles = mmax / 32;
-    bstep_diag_kernel<<<act_diag, 64, 0, s>>>(sys, order, k0, A, Dg, bad);
+    // step k0's reads: the original matrices at the first step (every element
+    // is written there, so A needs no restore), A after
+    const float* S = (k0 == 0 && A0) ? A0 : A;
+    bstep_diag_kernel<<<act_diag, 64, 0, s>>>(sys, order, k0, A, S, Dg, bad);
     if (act == 0) return hipGetLastError();
     const dim3 gs((unsigned)((mmax + 255) / 256), (unsigned)act);
     const dim3 gn((unsigned)((2 * tiles - 1 + 3) / 4), (unsigned)act);
     if (f16) {
-        bstep_strips_kernel<true><<<gs, 256, 0, s>>>(sys, order, k0, A, Dg, Lg, Tg, sstride);
-        bstep_update_kernel<true, 1><<<gn, 256, 0, s>>>(sys, order, k0, A, Lg, Tg, nullptr,
+        bstep_strips_kernel<true><<<gs, 256, 0, s>>>(sys, order, k0, A, S, Dg, Lg, Tg, sstride);
+        bstep_update_kernel<true, 1><<<gn, 256, 0, s>>>(sys, order, k0, A, S, Lg, Tg, nullptr,
                                                         nullptr, sstride);
     } else {
-        bstep_strips_kernel<false><<<gs, 256, 0, s>>>(sys, order, k0, A, Dg, Lg, Tg, sstride);
-        bstep_update_kernel<false, 1><<<gn, 256, 0, s>>>(sys, order, k0, A, Lg, Tg, nullptr,
+        bstep_strips_kernel<false><<<gs, 256, 0, s>>>(sys, order, k0, A, S, Dg, Lg, Tg, sstride);
+        bstep_update_kernel<false, 1><<<gn, 256, 0, s>>>(sys, order, k0, A, S, Lg, Tg, nullptr,
                                                          nullptr, sstride);
     }
-    // step k0 + 32 on the systems longer than it
-    bstep_diag_kernel<<<act, 64, 0, s>>>(sys, order, k0 + 32, A, Dg, bad);
+    // step k0 + 32 on the systems longer than it (its block and strips were
+    // written to A by the update above)
+    bstep_diag_kernel<<<act, 64, 0, s>>>(sys, order, k0 + 32, A, A, Dg, bad);
     if (act2 > 0 && tiles > 1) {
         const dim3 gs2((unsigned)((mmax - 32 + 255) / 256), (unsigned)act2);
         const dim3 gw((unsigned)(((tiles - 1) * (tiles - 1) + 3) / 4), (unsigned)act2);
         if (f16) {
-            bstep_strips_kernel<true><<<gs2, 256, 0, s>>>(sys, order, k0 + 32, A, Dg, Lg2, Tg2,
+            bstep_strips_kernel<true><<<gs2, 256, 0, s>>>(sys, order, k0 + 32, A, A, Dg, Lg2, Tg2,
                                                           sstride);
-            bstep_update_kernel<true, 2><<<gw, 256, 0, s>>>(sys, order, k0, A, Lg, Tg, Lg2, Tg2,
-                                                            sstride);
+            bstep_update_kernel<true, 2><<<gw, 256, 0, s>>>(sys, order, k0, A, S, Lg, Tg, Lg2,
+                                                            Tg2, sstride);
         } else {
-            bstep_strips_kernel<false><<<gs2, 256, 0, s>>>(sys, order, k0 + 32, A, Dg, Lg2, Tg2,
-                                                           sstride);
-            bstep_update_kernel<false, 2><<<gw, 256, 0, s>>>(sys, order, k0, A, Lg, Tg, Lg2, Tg2,
-                                                             sstride);
+            bstep_strips_kernel<false><<<gs2, 256, 0, s>>>(sys, order, k0 + 32, A, A, Dg, Lg2,
+                                                           Tg2, sstride);
+            bstep_update_kernel<false, 2><<<gw, 256, 0, s>>>(sys, order, k0, A, S, Lg, Tg, Lg2,
+                                                             Tg2, sstride);
         }
     }
     return hipGetLastError();

@@ -267,8 +267,10 @@ hipError_t launch_batch_step(bool f16, const BatchSystem* sys, const int* order,
 // 64.  Step k0 updates only the next step's diagonal block and strips; the
 // other tiles get both steps' updates in one pass (the trailing matrix is
 // read and written once for two steps), bit-identical to two single steps.
-// Step k0's strips in Lg / Tg, step k0 + 32's in Lg2 / Tg2.
+// Step k0's strips in Lg / Tg, step k0 + 32's in Lg2 / Tg2.  A0 non-null: at
+// k0 = 0 every read of the original elements comes from A0 (which then
+// needs no copy into A first).
 hipError_t launch_batch_step2(bool f16, const BatchSystem* sys, const int* order, int k0,
-                              int act_diag, int act, int act2, int mmax, float* A, float* Dg,
-                              int* bad, void* Lg, void* Tg, void* Lg2, void* Tg2, size_t sstride,
-                              hipStream_t s);
+                              int act_diag, int act, int act2, int mmax, float* A,
+                              const float* A0, float* Dg, int* bad, void* Lg, void* Tg, void* Lg2,
+                              void* Tg2, size_t sstride, hipStream_t s);
