@@ -461,44 +461,54 @@ __global__ __launch_bounds__(64) void block_inverse_kernel(const float* __restri
     const float* G = inv + k * inv_stride + (top ? 0 : NQ * 1024);
     float* o = out + k * 2 * (int64_t)B * B + (top ? (int64_t)B * B : 0);
 
+    // Every load is issued one block ahead of its use (the 32 x 32 inverse of
+    // the next block and the first coefficient block of the next column
+    // block while this one multiplies), and the stores' layout is selected by
+    // address, not by a branch, so the unrolled loop stays one stretch of code
+    // the compiler can schedule.
     f32x16 Y[NQ];
+    float g[2][16], c[2][16];
+    int cb = 0;  // buffer of c holding the next product's coefficients
+#pragma unroll
+    for (int r = 0; r < 16; ++r) g[0][r] = G[acc_row(r, lane) * 32 + lr];
+    if (NQ > 1) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) c[0][r] = D[(int64_t)acc_row(r, lane) * sp + (int64_t)(32 + lr) * si];
+    }
+    const int64_t ostep_i = top ? B : 1, ostep_v = top ? 1 : B;  // o offset per element / vector
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
+        if (q + 1 < NQ) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r)
+                g[(q + 1) & 1][r] = G[(q + 1) * 1024 + acc_row(r, lane) * 32 + lr];
+        }
         f32x16 acc;
 #pragma unroll
         for (int r = 0; r < 16; ++r) acc[r] = (q * 32 + acc_row(r, lane) == v0 + lr) ? 1.0f : 0.0f;
-        float c[2][16];
-        if (q > 0) {
-#pragma unroll
-            for (int r = 0; r < 16; ++r)
-                c[0][r] = D[(int64_t)acc_row(r, lane) * sp + (int64_t)(q * 32 + lr) * si];
-        }
 #pragma unroll
         for (int pb = 0; pb < q; ++pb) {
-            if (pb + 1 < q) {
+            // the coefficients of the next product: block pb + 1 of this
+            // column block, or block 0 of the next one
+            const int nq = pb + 1 < q ? q : q + 1, np = pb + 1 < q ? pb + 1 : 0;
+            if (nq < NQ) {
 #pragma unroll
                 for (int r = 0; r < 16; ++r)
-                    c[(pb + 1) & 1][r] = D[(int64_t)((pb + 1) * 32 + acc_row(r, lane)) * sp +
-                                           (int64_t)(q * 32 + lr) * si];
+                    c[cb ^ 1][r] = D[(int64_t)(np * 32 + acc_row(r, lane)) * sp +
+                                     (int64_t)(nq * 32 + lr) * si];
             }
 #pragma unroll
-            for (int r = 0; r < 16; ++r) acc = mfma32(c[pb & 1][r], Y[pb][r], acc);
+            for (int r = 0; r < 16; ++r) acc = mfma32(c[cb][r], Y[pb][r], acc);
+            cb ^= 1;
         }
-        const float* Gq = G + q * 1024;
         f32x16 y = {0};
 #pragma unroll
-        for (int r = 0; r < 16; ++r) y = mfma32(Gq[acc_row(r, lane) * 32 + lr], acc[r], y);
+        for (int r = 0; r < 16; ++r) y = mfma32(g[q & 1][r], acc[r], y);
         Y[q] = y;
         // element i = q*32 + acc_row(r) of vector v0 + lr
-        if (top) {
 #pragma unroll
-            for (int r = 0; r < 16; ++r)
-                o[(int64_t)(q * 32 + acc_row(r, lane)) * B + v0 + lr] = y[r];
-        } else {
-#pragma unroll
-            for (int r = 0; r < 16; ++r)
-                o[(int64_t)(v0 + lr) * B + q * 32 + acc_row(r, lane)] = y[r];
-        }
+        for (int r = 0; r < 16; ++r)
+            o[(int64_t)(q * 32 + acc_row(r, lane)) * ostep_i + (int64_t)(v0 + lr) * ostep_v] = y[r];
     }
 }
 
