@@ -43,7 +43,6 @@ __global__ __launch_bounds__(64) void prep_kernel(const float* __restrict__ A, i
                                                   int64_t off, int B, float* __restrict__ DT,
                                                   float* __restrict__ inv, int64_t inv_stride) {
     __shared__ float S[32][33];
-    __shared__ float X[32][33];  // X[i][j] = column j of the inverse
     const int NQ = B / 32;
     const int t = threadIdx.x;
     int blk = blockIdx.x;
@@ -84,39 +83,57 @@ __global__ __launch_bounds__(64) void prep_kernel(const float* __restrict__ A, i
     const int type = blk / NQ, q = blk % NQ;
     load_sub(q * 32, q * 32);
     __syncthreads();
-    if (t < 32) {
-        const int j = t;
-        if (type == 0) {
-            for (int i = 0; i < j; ++i) X[i][j] = 0.0f;
-            X[j][j] = S[j][j];
-            for (int i = j + 1; i < 32; ++i) {
-                float a = 0.0f;
-                for (int p = j; p < i; ++p) a = fmaf(S[i][p], X[p][j], a);
-                X[i][j] = a * S[i][i];
+    // Lane j (mod 32) computes column j of the inverse in registers, x[i] =
+    // X[i][j], with the same fmaf sequence per element as the textbook loops
+    // (a from 0, p = j .. i - 1 increasing; backward for type 2): the terms
+    // outside that range are computed and discarded by a select, so the
+    // accumulator stays +0 until the first real term.  The S reads have
+    // lane-independent addresses (LDS broadcasts) and do not depend on the
+    // chain, so only the fmaf chain is serial.
+    const int j = t & 31;
+    float x[32];
+    if (type == 0) {
+#pragma unroll
+        for (int i = 0; i < 32; ++i) {
+            float a = 0.0f;
+#pragma unroll
+            for (int p = 0; p < i; ++p) {
+                const float f = fmaf(S[i][p], x[p], a);
+                a = p >= j ? f : a;
             }
-        } else if (type == 1) {
-            for (int i = 0; i < j; ++i) X[i][j] = 0.0f;
-            X[j][j] = 1.0f;
-            for (int i = j + 1; i < 32; ++i) {
-                float a = 0.0f;
-                for (int p = j; p < i; ++p) a = fmaf(S[p][i], X[p][j], a);
-                X[i][j] = a;
+            const float sii = S[i][i];
+            x[i] = i > j ? a * sii : (i == j ? sii : 0.0f);
+        }
+    } else if (type == 1) {
+#pragma unroll
+        for (int i = 0; i < 32; ++i) {
+            float a = 0.0f;
+#pragma unroll
+            for (int p = 0; p < i; ++p) {
+                const float f = fmaf(S[p][i], x[p], a);
+                a = p >= j ? f : a;
             }
-        } else {
-            for (int i = j + 1; i < 32; ++i) X[i][j] = 0.0f;
-            X[j][j] = -S[j][j];
-            for (int i = j - 1; i >= 0; --i) {
-                float a = 0.0f;
-                for (int p = i + 1; p <= j; ++p) a = fmaf(S[p][i], X[p][j], a);
-                X[i][j] = a * S[i][i];
+            x[i] = i > j ? a : (i == j ? 1.0f : 0.0f);
+        }
+    } else {
+#pragma unroll
+        for (int i = 31; i >= 0; --i) {
+            float a = 0.0f;
+#pragma unroll
+            for (int p = i + 1; p < 32; ++p) {
+                const float f = fmaf(S[p][i], x[p], a);
+                a = p <= j ? f : a;
             }
+            const float sii = S[i][i];
+            x[i] = i < j ? a * sii : (i == j ? -sii : 0.0f);
         }
     }
-    __syncthreads();
-    float* out = inv + (int64_t)(type * NQ + q) * 1024;
-    for (int e = t; e < 1024; e += 64) {
-        const int j = e / 32, i = e % 32;  // out[j][i] = X[i][j]
-        out[e] = X[i][j];
+    // out[j][i] = X[i][j]: lane j writes its 32 values as 16-byte vectors
+    if (t < 32) {
+        float* out = inv + (int64_t)(type * NQ + q) * 1024 + j * 32;
+#pragma unroll
+        for (int i = 0; i < 32; i += 4)
+            *reinterpret_cast<float4*>(out + i) = float4{x[i], x[i + 1], x[i + 2], x[i + 3]};
     }
 }
 
