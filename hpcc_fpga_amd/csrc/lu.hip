@@ -83,15 +83,20 @@ __global__ __launch_bounds__(NT) void lu_diag_kernel(float* __restrict__ A, int6
             for (int r = 0; r < 32; ++r) d[r] = Ds[(P0 + r) * LD + P0 + c];
 #pragma unroll
             for (int j = 0; j < 32; ++j) {
+                // the lane's column, opaque to the compiler per step: the
+                // step's lane masks are formed here instead of all 64 being
+                // hoisted out of the panel loop and spilled
+                int cj = c;
+                asm volatile("" : "+v"(cj));
                 const float piv = bcast(d[j], j);
                 const float inv = -1.0f / piv;
                 bad |= (piv == 0.0f) || !__builtin_isfinite(inv);
-                d[j] = (c > j) ? d[j] * inv : ((c == j) ? inv : d[j]);
+                d[j] = (cj > j) ? d[j] * inv : ((cj == j) ? inv : d[j]);
 #pragma unroll
                 for (int r = j + 1; r < 32; ++r) {
                     const float sr = bcast(d[r], j);
                     const float u = d[r] + d[j] * sr;
-                    d[r] = (c > j) ? u : d[r];  // a select, not a branch
+                    d[r] = (cj > j) ? u : d[r];  // a select, not a branch
                 }
             }
             if (lane < 32) {
