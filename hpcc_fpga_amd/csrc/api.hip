@@ -307,22 +307,11 @@ hpl_status panel_strips(hpl_ctx* c, hipStream_t st, int64_t k, uint32_t B) {
             scT = c->dScT + slot * (n / 32) + (off + B) / 32;
             scL = c->dScL + slot * (n / 32) + (off + B) / 32;
         }
-        if (const float* t = step_tinv(c, k, B)) {
-            // the first B vectors of both strips -- all the next diagonal
-            // block reads -- in a launch of their own, marked by evH, so its
-            // update need not wait for the rest of the strips
-            const int64_t head = std::min<int64_t>(m, B);
-            HIPCHK(c, launch_trsm_inv(c->dA, lda, off, m, B, t, st, c->prec, lpT, lpL, B, 0,
-                                      head));
-            if (c->evH) HIPCHK(c, hipEventRecord(c->evH, st));
-            if (m > head)
-                HIPCHK(c, launch_trsm_inv(c->dA, lda, off, m, B, t, st, c->prec, lpT, lpL, B,
-                                          head, -1));
-        } else {
+        if (const float* t = step_tinv(c, k, B))
+            HIPCHK(c, launch_trsm_inv(c->dA, lda, off, m, B, t, st, c->prec, lpT, lpL, B));
+        else
             HIPCHK(c, launch_trsm(c->dA, lda, off, m, B, DT, inv, Lt, n, st, c->prec, lpT, lpL,
                                   B, scT, scL));
-            if (c->evH) HIPCHK(c, hipEventRecord(c->evH, st));
-        }
     }
     return HPL_SUCCESS;
 }
@@ -482,9 +471,7 @@ hpl_status gefa_pipelined(hpl_ctx* c, uint32_t B) {
             const int64_t from = ahead ? k - 1 : k0;
             const int K = (int)((k - from) * B);
             HIPCHK(c, hipEventRecord(c->evS, c->s0));
-            // the diagonal block needs only the head of the last strips (evH,
-            // recorded by panel_strips of step k-1)
-            HIPCHK(c, hipStreamWaitEvent(c->s2, c->evH, 0));
+            HIPCHK(c, hipStreamWaitEvent(c->s2, c->evS, 0));
             if (ahead) HIPCHK(c, hipStreamWaitEvent(c->s2, c->evL, 0));
             r = launch_update(c, c->s2, from, kb, kb, (int)B, (int)B, K, B);
             if (!r) r = panel_diag(c, c->s2, k, B);
@@ -672,8 +659,7 @@ hpl_status init_pipeline_streams(hpl_ctx* c) {
         hipStreamCreateWithFlags(&c->s3, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&c->evS, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&c->evD, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c->evL, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c->evH, hipEventDisableTiming) != hipSuccess)
+        hipEventCreateWithFlags(&c->evL, hipEventDisableTiming) != hipSuccess)
         return fail(c, HPL_ERR_HIP, "stream or event creation failed");
     return HPL_SUCCESS;
 }
@@ -838,7 +824,6 @@ void hpl_destroy(hpl_ctx* c) {
     if (c->evS) (void)hipEventDestroy(c->evS);
     if (c->evD) (void)hipEventDestroy(c->evD);
     if (c->evL) (void)hipEventDestroy(c->evL);
-    if (c->evH) (void)hipEventDestroy(c->evH);
     if (c->s3) (void)hipStreamDestroy(c->s3);
     if (c->s2) (void)hipStreamDestroy(c->s2);
     if (c->s1) (void)hipStreamDestroy(c->s1);

@@ -28,7 +28,6 @@ struct hpl_ctx {
     hipEvent_t evA = nullptr, evP = nullptr;  // next-panel region updated / panel ready
     hipEvent_t evS = nullptr, evD = nullptr;  // strips solved / diagonal block factorised
     hipEvent_t evL = nullptr;                 // next block's look-ahead update done
-    hipEvent_t evH = nullptr;                 // first B vectors of the last strips solved
     bool lookahead = true;
     // Pipelined schedule (with look-ahead): the bulk update of one group of
     // `depth` steps runs on s1 while the next group's panels run on s0; the

@@ -31,15 +31,13 @@ hipError_t launch_trsm(float* A, int64_t lda, int64_t off, int64_t m, int B, con
 // launch_trsm_inv then solves both strips as one product each, y = G v, on
 // BF16 MFMA with the operands split into high and low BF16 halves (three
 // products: about 2^-17 relative), writing the FP32 strips and their
-// k-contiguous 16-bit copies like launch_trsm (lp_prec 1: BF16; 2: FP16),
-// for vectors v_first .. v_first + v_count - 1 of both strips (v_count < 0:
-// the rest of the strip; both multiples of 32).
+// k-contiguous 16-bit copies like launch_trsm (lp_prec 1: BF16; 2: FP16).
 int64_t tinv_floats(int B);
 hipError_t launch_tinv_prepare(const float* A, int64_t lda, int64_t off, int B, const float* inv,
                                float* tinv, float* out, hipStream_t s);
 hipError_t launch_trsm_inv(float* A, int64_t lda, int64_t off, int64_t m, int B,
                            const float* tinv, hipStream_t s, int lp_prec, void* lpT, void* lpL,
-                           int64_t ldlp, int64_t v_first = 0, int64_t v_count = -1);
+                           int64_t ldlp);
 
 // the top strip alone (rows off..off+B-1, columns off+B..off+B+m-1), FP32, B <= 256
 hipError_t launch_trsm_top(float* A, int64_t lda, int64_t off, int64_t m, int B, const float* DT,

@@ -586,20 +586,18 @@ __global__ __launch_bounds__(64) void tinv_pack_kernel(const float* __restrict__
 }
 
 // One wave per 32 vectors: grid.x = ngroups_top top groups, then as many left
-// groups, covering vectors v_first .. v_first + 32 ngroups_top - 1 of both
-// strips.
+// groups.
 template <int B, typename LP>
 __global__ __launch_bounds__(64) void trsm_inv_kernel(float* __restrict__ A, int64_t lda,
                                                       int64_t off,
                                                       const bf16x8_t* __restrict__ pk,
-                                                      int ngroups_top, int64_t v_first,
-                                                      LP* __restrict__ lpT,
+                                                      int ngroups_top, LP* __restrict__ lpT,
                                                       LP* __restrict__ lpL, int64_t ldlp) {
     constexpr int NQ = B / 32, NB = NQ * (NQ + 1) / 2;
     const int lane = threadIdx.x;
     const int lr = lane & 31, lh = lane >> 5;
     const bool top = (int)blockIdx.x < ngroups_top;
-    const int64_t v0 = v_first + (int64_t)(top ? blockIdx.x : blockIdx.x - ngroups_top) * 32;
+    const int64_t v0 = (int64_t)(top ? blockIdx.x : blockIdx.x - ngroups_top) * 32;
     float* stripT = A + off * lda + off + B;
     float* stripL = A + (off + B) * lda + off;
     const bf16x8_t* G = pk + (top ? 0 : NB * 256) + lane;
@@ -816,25 +814,21 @@ hipError_t launch_tinv_prepare(const float* A, int64_t lda, int64_t off, int B, 
 
 hipError_t launch_trsm_inv(float* A, int64_t lda, int64_t off, int64_t m, int B,
                            const float* tinv, hipStream_t s, int lp_prec, void* lpT, void* lpL,
-                           int64_t ldlp, int64_t v_first, int64_t v_count) {
+                           int64_t ldlp) {
     if (m <= 0) return hipSuccess;
-    if (v_count < 0) v_count = m - v_first;
-    if (m % 32 || v_first % 32 || v_count % 32 || v_first < 0 || v_first + v_count > m ||
-        (lda & 3) || (off & 3) || (ldlp & 3) || (lp_prec != 1 && lp_prec != 2))
+    if (m % 32 || (lda & 3) || (off & 3) || (ldlp & 3) || (lp_prec != 1 && lp_prec != 2))
         return hipErrorInvalidValue;
-    if (v_count == 0) return hipSuccess;
-    const int groups = (int)(v_count / 32);
+    const int groups = (int)(m / 32);
     const bf16x8_t* pk = reinterpret_cast<const bf16x8_t*>(tinv + 2 * (int64_t)B * B);
     dim3 grid(2 * groups);
 #define HPL_TRSM_INV(BB)                                                                      \
     case BB:                                                                                  \
         if (lp_prec == 1)                                                                     \
-            trsm_inv_kernel<BB, __bf16><<<grid, 64, 0, s>>>(A, lda, off, pk, groups, v_first,  \
+            trsm_inv_kernel<BB, __bf16><<<grid, 64, 0, s>>>(A, lda, off, pk, groups,           \
                                                             static_cast<__bf16*>(lpT),         \
                                                             static_cast<__bf16*>(lpL), ldlp);  \
         else                                                                                  \
             trsm_inv_kernel<BB, _Float16><<<grid, 64, 0, s>>>(A, lda, off, pk, groups,         \
-                                                              v_first,                         \
                                                               static_cast<_Float16*>(lpT),     \
                                                               static_cast<_Float16*>(lpL),     \
                                                               ldlp);                           \
