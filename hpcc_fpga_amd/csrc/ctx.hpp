@@ -35,7 +35,7 @@ struct hpl_ctx {
     bool pipelined = true;
     bool lt_linear = false;
     // Column slices (separate launches) of each bulk update of the pipelined
-    // schedule; 0: as many as the update depth.
+    // schedule; 0: one per step of the group, at most 6 for the 16-bit update.
     int bulk_slices = 0;
     // Pipelined 16-bit schedule: CUs kept free of the bulk update for the
     // chain (0: bulk in column slices instead); num_cu = the device's CUs.
