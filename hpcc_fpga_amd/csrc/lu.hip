@@ -92,12 +92,16 @@ __global__ __launch_bounds__(NT) void lu_diag_kernel(float* __restrict__ A, int6
                 const float inv = -1.0f / piv;
                 bad |= (piv == 0.0f) || !__builtin_isfinite(inv);
                 d[j] = (cj > j) ? d[j] * inv : ((cj == j) ? inv : d[j]);
+                // the rows' products, then their sums, then the selects: three
+                // passes of independent operations (row by row, the compiler
+                // chains each row's three through one register)
+                float u[32];
 #pragma unroll
-                for (int r = j + 1; r < 32; ++r) {
-                    const float sr = bcast(d[r], j);
-                    const float u = d[r] + d[j] * sr;
-                    d[r] = (cj > j) ? u : d[r];  // a select, not a branch
-                }
+                for (int r = j + 1; r < 32; ++r) u[r] = d[j] * bcast(d[r], j);
+#pragma unroll
+                for (int r = j + 1; r < 32; ++r) u[r] = d[r] + u[r];
+#pragma unroll
+                for (int r = j + 1; r < 32; ++r) d[r] = (cj > j) ? u[r] : d[r];
             }
             if (lane < 32) {
 #pragma unroll
@@ -166,6 +170,7 @@ __global__ __launch_bounds__(NT) void lu_diag_kernel(float* __restrict__ A, int6
             for (int a = 0; a < 4; ++a)
 #pragma unroll
                 for (int b = 0; b < 4; ++b) acc[a][b] = Ds[(tj + a) * LD + ti + b];
+#pragma unroll 4
             for (int k = 0; k < 32; ++k) {
                 float lv[4], tv[4];
 #pragma unroll
