@@ -159,7 +159,19 @@ def _roofline(kind, launches, gemm_s, gemm_flops, gemm_bytes, busy_s, peak, traf
             "achieved_definition": "algorithmic flops per launch / average launch duration",
             "busy_s": round(busy_s, 6),
             "busy_union_achieved": round(device_achieved, 3),
-            "busy_union_frac": round(device_achieved / peak, 4)}
+            "busy_union_frac": round(device_achieved / peak, 4),
+            # The same kernel without the concurrent launches: rocprofv3's
+            # counter runs (the FETCH_SIZE pass behind `traffic`) serialise the
+            # dispatches, so their average duration is each launch's own time
+            # in the same schedule.  From that committed profile, not this run.
+            "serialised_achieved": round(tr["algorithmic_flops_per_launch"]
+                                         / (tr["avg_kernel_ns_counters"] * 1e-9) / 1e12, 3)
+            if tr else None,
+            "serialised_frac": round(tr["algorithmic_flops_per_launch"]
+                                     / (tr["avg_kernel_ns_counters"] * 1e-9) / 1e12 / peak, 4)
+            if tr else None,
+            "serialised_source": "profiles/" + traffic_file
+                                 + " (algorithmic_flops_per_launch / avg_kernel_ns_counters)"}
 
 
 def headline(ctx, n, B, d, steps, warmup, prec="bf16"):
