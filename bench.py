@@ -149,10 +149,11 @@ def _roofline(kind, launches, gemm_s, gemm_flops, gemm_bytes, busy_s, peak, traf
             "launches": launches, "avg_launch_us": round(gemm_s / max(1, launches) * 1e6, 1),
             "kernel_s": round(gemm_s, 6),
             # achieved = algorithmic flops per launch / average launch duration
-            # (HIP events on the launch streams).  The bulk update (stream s1)
-            # and the next panels' row and column-block updates (stream s0) run
-            # at the same time, so each launch's duration is stretched by the
-            # other stream's work and shared device time is counted once per
+            # (HIP events on the launch streams).  The bulk update (stream s1),
+            # the next panels' row and column-block updates (s2, and their
+            # look-ahead on s3) and the diagonal blocks' updates (s0) run at
+            # the same time, so each launch's duration is stretched by the
+            # other streams' work and shared device time is counted once per
             # launch.  busy_union_* divide the same flops by the time during
             # which at least one launch runs (the union of their intervals):
             # the device's update rate.

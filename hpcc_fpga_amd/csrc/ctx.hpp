@@ -22,11 +22,11 @@ struct hpl_ctx {
     bool external = false;
     hipStream_t s0 = nullptr;   // trailing updates (and everything else)
     hipStream_t s1 = nullptr;   // look-ahead panel factorisation
-    hipStream_t s2 = nullptr;   // diagonal blocks of the pipelined schedule
+    hipStream_t s2 = nullptr;   // row and column blocks' last update in the pipelined schedule
     hipStream_t s3 = nullptr;   // its look-ahead updates of the next block
     hipEvent_t e0 = nullptr, e1 = nullptr;
     hipEvent_t evA = nullptr, evP = nullptr;  // next-panel region updated / panel ready
-    hipEvent_t evS = nullptr, evD = nullptr;  // strips solved / diagonal block factorised
+    hipEvent_t evS = nullptr, evD = nullptr;  // strips solved / row and column blocks updated
     hipEvent_t evL = nullptr;                 // next block's look-ahead update done
     bool lookahead = true;
     // Pipelined schedule (with look-ahead): the bulk update of one group of
