@@ -415,7 +415,8 @@ hpl_status gefa_pipelined(hpl_ctx* c, uint32_t B) {
     const int64_t n = (int64_t)c->n;
     const int64_t N = n / B;
     const std::vector<int64_t> bnd =
-        hplx::group_bounds(c, N, B, c->depth, c->prec == HPL_PREC_FP32 ? 1 : 16);
+        hplx::group_bounds(c, N, B, c->depth, c->prec == HPL_PREC_FP32 ? 1 : 16,
+                           c->prec == HPL_PREC_BF16 || c->prec == HPL_PREC_FP16);
     const int64_t G = (int64_t)bnd.size() - 1;
     for (int64_t g = 0; g < G; ++g) {
         const int64_t s = bnd[g + 1] * (int64_t)B;
@@ -543,16 +544,20 @@ double elapsed_s(hpl_ctx* c) {
 namespace hplx {
 
 std::vector<int64_t> group_bounds(const hpl_ctx* c, int64_t N, int64_t B, int64_t depth,
-                                  int64_t tail_cap) {
+                                  int64_t tail_cap, bool deep_head) {
     const int64_t n = N * B;
     const int64_t tail_m = c->tail_m < 0 ? n / 2 : c->tail_m;
     const int64_t tail_d = c->tail_depth > 0
                                ? c->tail_depth
                                : std::max<int64_t>(1, std::min<int64_t>(depth / 2, tail_cap));
+    const int64_t head_m = c->head_m >= 0 ? c->head_m : deep_head ? n * 5 / 8 : 0;
+    const int64_t head_d = c->head_depth > 0 ? c->head_depth : deep_head ? depth + depth / 4 : 0;
     std::vector<int64_t> b{0};
     while (b.back() < N) {
-        const bool tail = tail_m > 0 && n - b.back() * B <= tail_m;
-        const int64_t d = std::max<int64_t>(1, tail ? tail_d : depth);
+        const int64_t left = n - b.back() * B;
+        const bool tail = tail_m > 0 && left <= tail_m;
+        const bool head = head_m > 0 && head_d > 0 && left > head_m;
+        const int64_t d = std::max<int64_t>(1, tail ? tail_d : head ? head_d : depth);
         b.push_back(std::min(N, b.back() + d));
     }
     return b;
@@ -792,6 +797,8 @@ hpl_status hpl_create(int device, hpl_ctx** out) {
         hipSuccess)
         c->num_cu = 0;
     if (const char* e = getenv("HPL_TAIL_M")) c->tail_m = std::max(0LL, atoll(e));
+    if (const char* e = getenv("HPL_HEAD_M")) c->head_m = std::max(0LL, atoll(e));
+    if (const char* e = getenv("HPL_HEAD_DEPTH")) c->head_depth = std::max(0, std::min(32, atoi(e)));
     if (const char* e = getenv("HPL_TAIL_DEPTH")) c->tail_depth = std::max(0, std::min(32, atoi(e)));
     if (const char* e = getenv("HPL_PANEL_SPIN_LIMIT")) c->pv_spin_limit = atol(e);
     if (const char* e = getenv("HPL_F64_DEPTH")) c->depth_f64 = std::max(1, std::min(16, atoi(e)));

@@ -49,6 +49,15 @@ struct hpl_ctx {
     // 65536 BF16 320 ms against 324.5.
     int64_t tail_m = -1;
     int tail_depth = 0;
+    // Head of the pipelined schedule for the BF16/FP16 update: while the order
+    // still to factorise exceeds head_m (-1: 5n/8; 0: no head), groups hold
+    // head_depth steps (0: 5/4 of the depth).  There the chain of a group
+    // finishes well before its bulk update, and deeper groups give fewer,
+    // longer bulk updates: n = 65536, depth 16, GEFA 239.3 / 240.7 / 239.9 ms
+    // without the head against 238.0 / 238.1 / 237.3 with 20 steps beyond
+    // order 40000 (A/B on one device; 24 or 32 steps were slower).
+    int64_t head_m = -1;
+    int head_depth = 0;
     // Steps per bulk trailing update (K = depth * B).  FP32: 3 -- at
     // n = 32768, B = 256 the bulk launches run at 0.75 of the FP32 MFMA peak
     // instead of 0.69-0.70 at depth 1, and with the pipelined schedule depth
@@ -215,9 +224,10 @@ uint64_t device_lda(uint64_t n);
 // b[0] = 0 < b[1] < ... = N, `depth` blocks per group until the order left
 // is at most the tail threshold, then the tail depth (the context's tail_m and
 // tail_depth, or their defaults: half the order, and half the depth but at
-// most tail_cap).
+// most tail_cap).  With deep_head, groups before that hold the head depth
+// while the order left exceeds the head threshold (head_m, head_depth).
 std::vector<int64_t> group_bounds(const hpl_ctx* c, int64_t N, int64_t B, int64_t depth,
-                                  int64_t tail_cap = 16);
+                                  int64_t tail_cap = 16, bool deep_head = false);
 
 // The full diagonal-block inverses of the resident factor (computed once per
 // factorisation, for blocks up to 256); dBInv stays null for larger blocks.
