@@ -372,16 +372,24 @@ hpl_status gefa_schedule(hpl_ctx* c, uint32_t B) {
 }
 
 // Pipelined schedule.  Steps form groups of D = depth; group g holds the
-// blocks [gD, gD + D).  Phase g runs two chains at once:
-//   s0: for each block k of group g, the update of its row block (columns
-//       kB..n) and column block (rows kB+B..n) by every step not yet applied
-//       there -- those of groups g-1 and g before k, one GEMM with
-//       K = (k - max(0, (g-1)D)) B over the contiguous strip slots -- then
-//       panel k (diagonal block and strip solves);
+// blocks [gD, gD + D).  Phase g runs at once:
 //   s1: the bulk update of group g-1's steps (K = D B) on the trailing
-//       matrix beyond group g's blocks, rows and columns >= (g+1) D B.
-// The two write disjoint parts of A and read only finished strips, so they
-// need no ordering inside the phase; at its end each stream waits for the
+//       matrix beyond group g's blocks, rows and columns >= (g+1) D B;
+//   for each block k of group g, which still needs the steps of group g-1
+//   and those of group g before k:
+//   s0: the diagonal block's update by those steps, its factorisation and
+//       the strip solves -- the critical path, on one stream;
+//   s2: the update of the row block (columns kB+B..n) and column block (rows
+//       kB+B..n) by the same steps, beside the factorisation (the strip
+//       solves wait for it);
+//   s3: the look-ahead: block k+1's diagonal, row and column blocks updated
+//       by all its pending steps but step k, so that at step k+1 only one
+//       step (K = B) is left on s0 and s2 (the group's first block takes
+//       all of its pending steps there, K = (k - max(0, (g-1)D)) B over the
+//       contiguous strip slots).
+// The streams write disjoint parts of A (or wait for each other where they
+// do not: the look-ahead before the next step's updates of the same block)
+// and read only finished strips; at the phase's end s1 and s0 wait for each
 // other (s1 for the panels whose strips the next bulk update reads, s0 for
 // the bulk update that covers the next group's blocks).  Every element still
 // receives each step's update exactly once, in step order.
@@ -394,10 +402,13 @@ hpl_status gefa_schedule(hpl_ctx* c, uint32_t B) {
 // the chain advance.  With as many slices as steps per group, n = 32768,
 // B = 256, D = 3 measured 207 ms against 225 ms for the grouped schedule and
 // 221 ms for this one with single bulk launches (D = 4: 207 ms, D = 2:
-// 210 ms; 2 slices at D = 3: 215 ms).
+// 210 ms; 2 slices at D = 3: 215 ms).  The 16-bit update takes at most 6
+// slices per group (below).
 //
 // Groups need not be equal: hplx::group_bounds gives their boundaries, so the
-// last part of the factorisation can use shorter groups (tail_m, tail_depth).
+// last part of the factorisation can use shorter groups (tail_m, tail_depth)
+// and, for the 16-bit update, the first part deeper ones (head_m,
+// head_depth).
 // FP32 defaults to D = 4 with single steps in the tail (n = 32768: GEFA
 // 203.6-204.9 ms against 204.2-207.9 for D = 3; D = 4 with tail groups of 2:
 // 204.3-205.1; D = 5 and 6 with single steps: 203.8-205.6, 204.7-205.9).
