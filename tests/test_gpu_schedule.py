@@ -76,11 +76,18 @@ def test_pipelined_equals_grouped_fp32(n, B, depth, slices, tail):
     assert np.array_equal(g.view(np.uint32), p.view(np.uint32))
 
 
-@pytest.mark.parametrize("prec", ["bf16", "fp8"])
-def test_pipelined_equals_grouped_low_precision(prec):
-    n, B = 3072, 256
-    g = _factor({"HPL_SCHEDULE": "grouped", "HPL_UPDATE_DEPTH": "5"}, n, B, prec)
-    p = _factor({"HPL_SCHEDULE": "pipelined", "HPL_UPDATE_DEPTH": "5"}, n, B, prec)
+@pytest.mark.parametrize("prec,n,depth,extra", [
+    ("bf16", 3072, 5, {}),  # the default head (groups of 6 above 5n/8) and tail
+    ("fp8", 3072, 5, {}),
+    # two head groups of 6 blocks, then the tail's groups of 2: 16 blocks
+    ("bf16", 4096, 4, {"HPL_HEAD_M": "2048", "HPL_HEAD_DEPTH": "6"}),
+])
+def test_pipelined_equals_grouped_low_precision(prec, n, depth, extra):
+    B = 256
+    g = _factor({"HPL_SCHEDULE": "grouped", "HPL_UPDATE_DEPTH": str(depth)}, n, B, prec)
+    env = {"HPL_SCHEDULE": "pipelined", "HPL_UPDATE_DEPTH": str(depth)}
+    env.update(extra)
+    p = _factor(env, n, B, prec)
     assert np.array_equal(g.view(np.uint32), p.view(np.uint32))
 
 
