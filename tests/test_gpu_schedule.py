@@ -24,7 +24,7 @@ from hpcc_fpga_amd import Context
 pytestmark = pytest.mark.gpu
 
 KEYS = ("HPL_SCHEDULE", "HPL_UPDATE_DEPTH", "HPL_BULK_SLICES", "HPL_F64_DEPTH", "HPL_TAIL_M",
-        "HPL_TAIL_DEPTH")
+        "HPL_TAIL_DEPTH", "HPL_HEAD_M", "HPL_HEAD_DEPTH")
 
 
 def _context(env):
